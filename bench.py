@@ -1,0 +1,127 @@
+"""Benchmark of the per-step phase-space update on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): plain VFP,
+one 512x512 FP32 mesh, kick-drift scheme (wake kick, sinusoidal RF kick,
+linear drift, Fokker-Planck), parallel-plates CSR impedance with a 32 mm
+gap, FFT length 4096, current 1.5x the scaling-law threshold, 2000 steps per
+synchrotron period; the bunch profile and the CSR power are recorded on the
+device every step.  One "step" is one time step of the mesh.
+
+Prints one JSON line.  value = mesh-cell updates per second
+(n^2 * steps / wall time of the timed steps, state resident on the GPU).
+roofline: the fused map kernel (8 bytes of mesh traffic per cell) timed
+with HIP events on its stream.  cpu_baseline: the oracle (the FP32 C
+restatement of the reference CPU path) on one host core, on a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("mesh-cell updates/s at 512²; batched+surrogate threshold scan "
+          "time vs plain @2%")
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(desc, zw, zc, init, budget_s=15.0):
+    import numpy as np
+    from oracle.oracle import Oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _helpers import oracle_cfg
+    o = Oracle()
+    sim = o.sim(oracle_cfg(desc, zw.n), init, zw.data(), zc.data())
+    t0 = time.perf_counter()
+    sim.run(3, want_csr=True)
+    per = (time.perf_counter() - t0) / 3
+    steps = max(3, min(2000, int(budget_s / per)))
+    t0 = time.perf_counter()
+    sim.run(steps, want_csr=True)
+    dt = time.perf_counter() - t0
+    n = desc["n"]
+    return {"value": n * n * steps / dt, "unit": "cell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle FP32 restatement, {steps} steps of the same "
+                      f"{n}x{n} kick-drift workload from the same initial "
+                      f"state, one thread ({dt:.1f} s)"}
+
+
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
+              "not supported (independent instances shard across GPUs by "
+              "process, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    from inovesa_amd import Step
+    from inovesa_amd.workloads import impedances, initial_mesh, kick_drift_config
+
+    n = args.n
+    desc, d = kick_drift_config(n=n, steps_per_ts=2000, gap=0.032,
+                                current_factor=1.5)
+    zw, zc = impedances(d)
+    init = initial_mesh(n, seed=1)
+    st = Step(desc, init, zw, zc)
+    st.record(args.warmup + args.steps, profiles=True)
+    st.run(args.warmup)
+    st.dev.sync()
+    t0 = time.perf_counter()
+    st.run(args.steps)
+    st.dev.sync()
+    dt = time.perf_counter() - t0
+    cnt_csr, _ = st.history()
+    assert len(cnt_csr) == args.warmup + args.steps
+    value = n * n * args.steps / dt
+
+    ms_wake, ms_maps, _ = st.timing(200)
+    bytes_per_launch = 8.0 * n * n
+    achieved = bytes_per_launch / (ms_maps * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": load_traffic(f"maps_kd_{n}"),
+                "kernel": "maps_kernel (wake+RF kick, drift, Fokker-Planck, "
+                          "projection)",
+                "kernel_ms": ms_maps, "field_kernel_ms": ms_wake}
+
+    out = {"metric": METRIC, "value": value, "unit": "cell-updates/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded Gaussian + 1e-4 uniform noise)",
+           "config": {"workload": f"plain VFP {n}x{n} kick-drift, PP-CSR "
+                                  "gap 32 mm, nmax "
+                                  f"{zw.n}, 1.5x scaling-law threshold, "
+                                  "2000 steps/Ts, profile + CSR power "
+                                  "recorded every step",
+                      "n": n, "nmax": zw.n, "steps_per_ts": 2000,
+                      "Ib_A": d["Ib"], "tile_rows": int(desc.get("tile_rows", 0))},
+           "roofline": roofline}
+    if not args.no_cpu:
+        cb = cpu_baseline(desc, zw, zc, init, args.cpu_budget)
+        cb["ratio_gpu_over_cpu"] = value / cb["value"]
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

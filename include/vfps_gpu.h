@@ -1,0 +1,346 @@
+/*
+ * vfps_gpu.h -- C ABI of the MI355X (gfx950) VFP phase-space step.
+ *
+ * The library replaces the device back end of Inovesa's per-step update:
+ * the OpenCL handle `oclhptr_t` (inc/CL/OpenCLHandler.hpp:36-41) that every
+ * PhaseSpace, SourceMap, ElectricField and Impedance constructor receives.
+ * Each entry point below names the reference interface it stands for.
+ *
+ * Conventions
+ *   - Every call returns 0 on success and a negative vfps_status on failure;
+ *     vfps_last_error() gives a message (thread-local).  No C++ exception
+ *     crosses this boundary.
+ *   - One HIP stream per device handle; calls are asynchronous on it, except
+ *     those that return data to host memory, which synchronise.
+ *   - Host pointers are borrowed for the duration of the call only.  Device
+ *     memory is owned by the handles.
+ *   - Meshes are FP32, layout [bunch][x][y] row-major, y contiguous
+ *     (inc/PS/PhaseSpace.hpp:444-447).
+ */
+#ifndef VFPS_GPU_H
+#define VFPS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFPS_API __attribute__((visibility("default")))
+
+typedef enum {
+    VFPS_OK = 0,
+    VFPS_ERR_INVALID = -1,  /* bad argument (std::invalid_argument in the reference) */
+    VFPS_ERR_HIP = -2,      /* HIP runtime error */
+    VFPS_ERR_NOMEM = -3,
+    VFPS_ERR_UNSUPPORTED = -4
+} vfps_status;
+
+typedef struct vfps_dev vfps_dev;
+typedef struct vfps_mesh vfps_mesh;
+typedef struct vfps_map vfps_map;
+typedef struct vfps_impedance vfps_impedance;
+typedef struct vfps_field vfps_field;
+typedef struct vfps_step vfps_step;
+typedef struct vfps_batch vfps_batch;
+typedef struct vfps_mlp vfps_mlp;
+
+typedef struct {
+    float re;
+    float im;
+} vfps_cplx;
+
+/* ---- errors, version ---------------------------------------------------- */
+VFPS_API const char* vfps_last_error(void);
+VFPS_API const char* vfps_version(void);
+
+/* ---- device (replaces OCLH, inc/CL/OpenCLHandler.hpp:63-314) ------------ */
+VFPS_API int vfps_dev_open(int ordinal, vfps_dev** out);
+VFPS_API int vfps_dev_close(vfps_dev* dev);
+/* OCLH::flush/finish (src/main.cpp:1046-1050) */
+VFPS_API int vfps_sync(vfps_dev* dev);
+
+/* ---- PhaseSpace (inc/PS/PhaseSpace.hpp:40-552) -------------------------- */
+/* PhaseSpace(qmin,qmax,qscale,pmin,pmax,pscale,oclh,Qb,Ib,filling,zoom,data)
+ * (src/PS/PhaseSpace.cpp:149-168); nx = ny = n, nb = number of bunches.
+ * data == NULL builds the Gaussian of src/PS/PhaseSpace.cpp:56-63 on the
+ * device; otherwise the mesh is copied from data ([nb][n][n]).
+ * Fails with VFPS_ERR_INVALID if filling does not sum to 1 (:47-51). */
+VFPS_API int vfps_mesh_create(vfps_dev* dev, uint32_t nb, uint32_t n,
+                              float qmin, float qmax, double qscale,
+                              float pmin, float pmax, double pscale,
+                              double charge, double current,
+                              const float* filling, double zoom,
+                              const float* data, vfps_mesh** out);
+/* copy constructor (src/PS/PhaseSpace.cpp:170-180) */
+VFPS_API int vfps_mesh_clone(const vfps_mesh* src, vfps_mesh** out);
+VFPS_API int vfps_mesh_destroy(vfps_mesh* m);
+/* syncCLMem(cpu2dev) / syncCLMem(dev2cpu) (src/PS/PhaseSpace.cpp:381-414) */
+VFPS_API int vfps_mesh_upload(vfps_mesh* m, const float* data);
+VFPS_API int vfps_mesh_download(vfps_mesh* m, float* data);
+/* operator= / swap (src/PS/PhaseSpace.cpp:374-379, 503-506) */
+VFPS_API int vfps_mesh_copy(vfps_mesh* dst, const vfps_mesh* src);
+/* getData(): device pointer, valid while the mesh lives */
+VFPS_API int vfps_mesh_device_ptr(vfps_mesh* m, float** dptr);
+/* updateXProjection (src/PS/PhaseSpace.cpp:291-323) */
+VFPS_API int vfps_mesh_project_x(vfps_mesh* m);
+/* updateYProjection (src/PS/PhaseSpace.cpp:325-340) */
+VFPS_API int vfps_mesh_project_y(vfps_mesh* m);
+/* getProjection(axis) ([nb][n]); synchronises */
+VFPS_API int vfps_mesh_get_projection(vfps_mesh* m, int axis, float* out);
+/* setProjection(axis, bunch, ...) (inc/PS/PhaseSpace.hpp:197-207) */
+VFPS_API int vfps_mesh_set_projection(vfps_mesh* m, int axis,
+                                      const float* proj);
+/* integrate (src/PS/PhaseSpace.cpp:200-239), accumulated in FP64.
+ * filling (nb) and integral may be NULL; non-NULL outputs synchronise. */
+VFPS_API int vfps_mesh_integrate(vfps_mesh* m, double* filling,
+                                 double* integral);
+/* normalize (src/PS/PhaseSpace.cpp:342-372) with the last integrate() */
+VFPS_API int vfps_mesh_normalize(vfps_mesh* m);
+/* variance(axis) (src/PS/PhaseSpace.cpp:241-289): mean, variance and rms
+ * per bunch from the current projection of that axis; synchronises */
+VFPS_API int vfps_mesh_variance(vfps_mesh* m, int axis, float* mean,
+                                float* var, float* rms);
+/* geometry: getDelta(axis), zerobin, Simpson weights (n) */
+VFPS_API int vfps_mesh_info(const vfps_mesh* m, uint32_t* nb, uint32_t* n,
+                            float* delta0, float* delta1, float* zerobin0,
+                            float* zerobin1);
+VFPS_API int vfps_mesh_axis(const vfps_mesh* m, int axis, float* out);
+VFPS_API int vfps_mesh_simpson_weights(const vfps_mesh* m, float* out);
+
+/* ---- SourceMaps (inc/SM/SourceMap.hpp:53-87) ---------------------------- */
+/* SourceMap::InterpolationType (inc/SM/SourceMap.hpp:22-27) */
+enum { VFPS_INTERP_NONE = 1, VFPS_INTERP_LINEAR = 2,
+       VFPS_INTERP_QUADRATIC = 3, VFPS_INTERP_CUBIC = 4 };
+/* KickMap(in,out,it,interpol_clamp,kd,oclh) (src/SM/KickMap.cpp:25-192);
+ * axis 0 = Axis::x (displacement along x, per y), 1 = Axis::y.
+ * The reference CPU path ignores interpol_clamp for kick maps; so does
+ * this one. */
+VFPS_API int vfps_map_create_kick(vfps_mesh* in, vfps_mesh* out, int interp,
+                                  int clamp, int axis, vfps_map** map);
+/* KickMap::swapOffset (inc/SM/KickMap.hpp:51-55): offsets in mesh units,
+ * n entries (pd * nb) */
+VFPS_API int vfps_map_set_offset(vfps_map* map, const float* offset,
+                                 size_t n);
+/* KickMap::getForce (inc/SM/KickMap.hpp:47-49); synchronises */
+VFPS_API int vfps_map_get_offset(vfps_map* map, float* offset, size_t n);
+/* DriftMap(in,out,slip,E0,it,clamp,oclh) (src/SM/DriftMap.cpp:11-38) */
+VFPS_API int vfps_map_create_drift(vfps_mesh* in, vfps_mesh* out,
+                                   const float* slip, uint32_t nslip,
+                                   float E0, int interp, int clamp,
+                                   vfps_map** map);
+/* RFKickMap linear (src/SM/RFKickMap.cpp:15-34) */
+VFPS_API int vfps_map_create_rfkick_linear(vfps_mesh* in, vfps_mesh* out,
+                                           float angle, float f_RF,
+                                           int interp, int clamp,
+                                           vfps_map** map);
+/* RFKickMap sinusoidal (src/SM/RFKickMap.cpp:36-57) */
+VFPS_API int vfps_map_create_rfkick(vfps_mesh* in, vfps_mesh* out,
+                                    float revolutionpart, float V_RF,
+                                    float f_RF, float V0, int interp,
+                                    int clamp, vfps_map** map);
+/* RotationMap(in,out,xsize,ysize,angle,it,clamped,rotmapsize,oclh)
+ * (src/SM/RotationMap.cpp:12-95).  The taps are computed on the fly for
+ * any rotmapsize; rotmapsize only selects the reference semantics (clamping
+ * needs cubic interpolation and rotmapsize > 0, else VFPS_ERR_INVALID). */
+VFPS_API int vfps_map_create_rotation(vfps_mesh* in, vfps_mesh* out,
+                                      float angle, int interp, int clamp,
+                                      uint32_t rotmapsize, vfps_map** map);
+/* FokkerPlanckMap(in,out,xsize,ysize,fptype,fptrack,e1,dt,oclh)
+ * (src/SM/FokkerPlanckMap.cpp:11-118).  fptype: 0 none, 1 damping only,
+ * 2 diffusion only, 3 full; deriv: 3 two-sided, 4 cubic, 5 five-point
+ * (this library's extension, no reference counterpart). */
+VFPS_API int vfps_map_create_fokkerplanck(vfps_mesh* in, vfps_mesh* out,
+                                          int fptype, int fptrack, float e1,
+                                          int deriv, vfps_map** map);
+/* Identity (inc/SM/Identity.hpp:14-61) */
+VFPS_API int vfps_map_create_identity(vfps_mesh* in, vfps_mesh* out,
+                                      vfps_map** map);
+/* WakePotentialMap(in,out,field,it,clamp,oclh)
+ * (src/SM/WakePotentialMap.cpp:11-25) */
+VFPS_API int vfps_map_create_wakepotential(vfps_mesh* in, vfps_mesh* out,
+                                           vfps_field* field, int interp,
+                                           int clamp, vfps_map** map);
+/* WakePotentialMap::update (src/SM/WakePotentialMap.cpp:34-50): wake
+ * potential of the field's phase space becomes this map's offset, on the
+ * device */
+VFPS_API int vfps_map_update(vfps_map* map);
+/* SourceMap::apply (src/SM/SourceMap.cpp:75-109 and overrides) */
+VFPS_API int vfps_map_apply(vfps_map* map);
+VFPS_API int vfps_map_destroy(vfps_map* map);
+
+/* ---- Impedance (inc/Z/Impedance.hpp; src/Z/*) --------------------------- */
+/* Impedance(nfreqs, f_max): a zero table (src/Z/Impedance.cpp:48-57) */
+VFPS_API int vfps_impedance_create(vfps_dev* dev, uint32_t nfreqs,
+                                   float f_max, vfps_impedance** out);
+/* Impedance(z, f_max) from a table (src/Z/Impedance.cpp:35-46); also the
+ * result of Impedance(datafile, ...) once the file is read */
+VFPS_API int vfps_impedance_create_table(vfps_dev* dev, const vfps_cplx* z,
+                                         uint32_t nfreqs, float f_max,
+                                         vfps_impedance** out);
+VFPS_API int vfps_impedance_destroy(vfps_impedance* z);
+/* operator+= of the named contribution, each computed on the device in
+ * FP64 and rounded to complex<float>:
+ *   FreeSpaceCSR(n, f_rev, f_max)          src/Z/FreeSpaceCSR.cpp:19-42
+ *   ParallelPlatesCSR(n, f0, f_max, g)     src/Z/ParallelPlatesCSR.cpp:19-69
+ *   ResistiveWall(n, f0, f_max, L, s, xi, b) src/Z/ResistiveWall.cpp:25-55
+ *   ConstImpedance(n, f_max, Z)            src/Z/ConstImpedance.cpp:11-27
+ *   CollimatorImpedance(n, f_max, outer, inner) src/Z/CollimatorImpedance.cpp */
+VFPS_API int vfps_impedance_add_freespace_csr(vfps_impedance* z, float f_rev,
+                                              float f_max);
+VFPS_API int vfps_impedance_add_parallel_plates(vfps_impedance* z, float f0,
+                                                float f_max, double g);
+VFPS_API int vfps_impedance_add_resistive_wall(vfps_impedance* z, float f0,
+                                               float f_max, double L,
+                                               double s, double xi, double b);
+VFPS_API int vfps_impedance_add_const(vfps_impedance* z, float re, float im);
+VFPS_API int vfps_impedance_add_collimator(vfps_impedance* z, double outer,
+                                           double inner);
+VFPS_API int vfps_impedance_add(vfps_impedance* z, const vfps_impedance* rhs);
+/* operator[] over the whole table; synchronises */
+VFPS_API int vfps_impedance_download(vfps_impedance* z, vfps_cplx* out);
+VFPS_API int vfps_impedance_nfreqs(const vfps_impedance* z, uint32_t* n);
+/* FP64 Airy functions Ai, Ai', Bi, Bi' on the device for x >= 0 (the
+ * kernel the ParallelPlatesCSR table uses); out: 4*n doubles, ovf: n flags
+ * set where Bi or Bi' exceed the double range; synchronises */
+VFPS_API int vfps_airy(vfps_dev* dev, const double* x, uint32_t n,
+                       double* out, int32_t* ovf);
+
+/* ---- ElectricField (inc/PS/ElectricField.hpp) --------------------------- */
+/* ElectricField(ps, impedance, buckets, spacing_bins, oclh, f_rev,
+ * revolutionpart, wakescaling) (src/PS/ElectricField.cpp:14-101).
+ * wakescaling is the constructor argument; the field divides it by nmax
+ * (inc/PS/ElectricField.hpp:99-100).  Pass 0 for a radiation-only field. */
+VFPS_API int vfps_field_create(vfps_mesh* ps, vfps_impedance* z,
+                               const uint32_t* buckets, uint32_t nbuckets,
+                               uint32_t spacing_bins, double f_rev,
+                               float revolutionpart, float wakescaling,
+                               vfps_field** out);
+/* the scaling argument of the second constructor
+ * (inc/PS/ElectricField.hpp:81-103): Ib*dt*c/Meter/(dp*sigma_delta*E0) */
+VFPS_API float vfps_field_wakescaling(double Ib, double dt, double qscale,
+                                      float delta1, double sigma_delta,
+                                      double E0);
+VFPS_API int vfps_field_destroy(vfps_field* f);
+/* padBunchProfiles (src/PS/ElectricField.cpp:255-263) */
+VFPS_API int vfps_field_pad(vfps_field* f);
+/* getPaddedBunchProfiles: nmax floats; synchronises */
+VFPS_API int vfps_field_get_padded(vfps_field* f, float* out);
+/* wakePotential (src/PS/ElectricField.cpp:186-253); wp ([nb][n]) may be
+ * NULL (the result stays on the device); non-NULL synchronises */
+VFPS_API int vfps_field_wake_potential(vfps_field* f, float* wp);
+/* getWakeScaling (inc/PS/ElectricField.hpp:306-307) */
+VFPS_API int vfps_field_get_wakescaling(const vfps_field* f, float* s);
+/* updateCSR(fc) (src/PS/ElectricField.cpp:133-184); spectrum ([nb][nmax])
+ * and power (nb) may be NULL; non-NULL synchronises */
+VFPS_API int vfps_field_update_csr(vfps_field* f, float fc, float* power,
+                                   float* spectrum);
+
+/* ---- ring parameters (src/main.cpp:179-355) ----------------------------- */
+typedef struct {
+    uint32_t grid_size;       /* GridSize */
+    double pq_size;           /* PhaseSpaceSize */
+    double shift_x, shift_y;  /* PhaseSpaceShiftX/Y */
+    double energy_spread;     /* sE */
+    double beam_energy;       /* E0 (eV) */
+    double bending_radius;    /* R; <= 0: from f_rev */
+    double f_rev;             /* RevolutionFrequency */
+    double harmonic_number;
+    double rf_voltage;        /* V_RF */
+    double f_s;               /* SyncFreq; 0: from alpha0 */
+    double alpha0, alpha1, alpha2;
+    double vacuum_gap;        /* VacuumChamberGap (m), <0 free space */
+    double damping_time;      /* <0: from ring parameters */
+    double padding;
+    int32_t round_padding;
+    uint32_t steps_per_ts;
+    double rotations;
+    double bunch_current;     /* single bunch */
+    double cutoff_frequency;  /* fc */
+} vfps_ring_params;
+
+typedef struct {
+    float qmin, qmax, pmin, pmax;
+    double qscale;            /* bunch length bl (m): "Meter" scale */
+    double pscale;            /* dE (eV): "ElectronVolt" scale */
+    double R_bend, f0;        /* f0 = c/(2 pi R) */
+    double V0, V_eff, f_s, alpha0;
+    double f_RF;
+    double steps, dt, revolutionpart;
+    float angle;              /* 2 pi / steps */
+    float slip[3];
+    float e1;                 /* 2/(fs t_damp steps) */
+    double t_damp;
+    uint32_t padded_bins;     /* FFT length nmax */
+    uint32_t spacing_bins;
+    float fmax;
+    double Ib;
+    float wakescaling;        /* ElectricField scaling argument */
+    double shield;            /* shielding parameter, g = gap */
+    double Inorm;             /* scaling-law normalisation current */
+    double Ith;               /* scaling-law threshold Inorm*(0.5+0.34*shield) */
+    uint32_t laststep;
+} vfps_derived;
+
+VFPS_API int vfps_derive(const vfps_ring_params* p, vfps_derived* d);
+
+/* ---- whole step on one mesh (src/main.cpp:948-1053) --------------------- *
+ * A step object owns three meshes (grid_t1..t3), the maps and the fields;
+ * run() advances nsteps steps with the wake of each step taken from the
+ * profile of the previous one, exactly like the reference loop. */
+typedef struct {
+    uint32_t n;
+    float qmin, qmax, pmin, pmax;
+    double qscale, pscale;
+    uint32_t scheme;          /* 0: wake, RF kick, drift, FP; 1: wake, rotation, FP */
+    uint32_t interp;          /* interpolation points (1..4) */
+    int32_t rot_clamp;
+    float angle;              /* scheme 1: rotation angle per step */
+    int32_t rf_linear;        /* scheme 0: RFKickMap flavour */
+    float rf_angle;           /* linear RF kick angle */
+    float revolutionpart, V_RF, f_RF, V0;
+    float slip[3];
+    uint32_t nslip;
+    float E0;
+    int32_t use_fp;
+    uint32_t fptype;
+    float e1;
+    uint32_t deriv;
+    int32_t use_wake;
+    uint32_t spacing_bins;
+    float wakescaling;        /* ElectricField scaling argument */
+    int32_t use_csr;
+    float fc;
+    int32_t fused;            /* 1: fused kernels (default), 0: one kernel per map */
+    uint32_t tile_rows;       /* 0: autotuned; rows per workgroup of the fused kernel */
+} vfps_step_desc;
+
+VFPS_API int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
+                              const float* init, vfps_impedance* zwake,
+                              vfps_impedance* zcsr, vfps_step** out);
+VFPS_API int vfps_step_destroy(vfps_step* s);
+/* Advance nsteps.  The step is captured once into a hipGraph and replayed.
+ * Per-step CSR power and bunch profiles are kept on the device
+ * (vfps_step_get_history). */
+VFPS_API int vfps_step_run(vfps_step* s, uint32_t nsteps);
+/* record per-step outputs of the next run() calls: capacity steps of CSR
+ * power (and profiles if want_profiles) */
+VFPS_API int vfps_step_record(vfps_step* s, uint32_t capacity,
+                              int want_profiles);
+VFPS_API int vfps_step_get_history(vfps_step* s, uint32_t* count,
+                                   float* csr_power, float* profiles);
+VFPS_API int vfps_step_get_mesh(vfps_step* s, float* mesh, float* proj,
+                                float* wake);
+VFPS_API int vfps_step_set_mesh(vfps_step* s, const float* mesh);
+/* time the last run(): average duration of each kernel of the step, from
+ * HIP events on the step's stream (milliseconds); names are
+ * "wake", "maps" */
+VFPS_API int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
+                              float* ms_maps, float* ms_step);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,324 @@
+// ElectricField (inc/PS/ElectricField.hpp, src/PS/ElectricField.cpp):
+// padded bunch profile -> real FFT -> x Z(n) -> inverse real FFT -> wake
+// potential, and the CSR spectrum/power from the same forward transform.
+//
+// One workgroup runs a whole transform pair in LDS (fft.hpp); the complex
+// multiply by Z, the Hermitian post/pre passes, the scaling and the CSR
+// spectrum are fused into the pass between the two FFTs.
+#include <cmath>
+
+#include "common.hpp"
+#include "field_body.hpp"
+#include "maps.hpp"
+#include "reduce.hpp"
+
+namespace vfps_impl {
+
+template <bool WAKE, bool CSR>
+__global__ void __launch_bounds__(512)
+field_kernel(FieldParams p)
+{
+    extern __shared__ float2 lds[];
+    field_body<WAKE, CSR>(p, blockIdx.x, lds);
+}
+
+size_t field_lds_bytes(uint32_t M) { return sizeof(float2) * 2 * (size_t)M; }
+
+int launch_field(const FieldParams& p, bool wake, bool csr, uint32_t blocks,
+                 hipStream_t st)
+{
+    const size_t lds = field_lds_bytes(p.M);
+    if (wake && csr) {
+        hipLaunchKernelGGL((field_kernel<true, true>), dim3(1), dim3(512), lds,
+                           st, p);
+    } else if (wake) {
+        hipLaunchKernelGGL((field_kernel<true, false>), dim3(1), dim3(512),
+                           lds, st, p);
+    } else {
+        hipLaunchKernelGGL((field_kernel<false, true>), dim3(blocks),
+                           dim3(512), lds, st, p);
+    }
+    VFPS_HIP(hipGetLastError());
+    return VFPS_OK;
+}
+
+int field_set_lds_limit(size_t bytes)
+{
+    if (bytes <= 65536) {
+        return VFPS_OK;
+    }
+    VFPS_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&field_kernel<true, true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    VFPS_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&field_kernel<true, false>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    VFPS_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&field_kernel<false, true>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return VFPS_OK;
+}
+
+// twiddle tables: e^{-2 pi i t/M} (t < M) and e^{-2 pi i k/N} (k < M),
+// computed in double, rounded to float
+void make_twiddles(uint32_t M, std::vector<float2>& tw_m,
+                   std::vector<float2>& tw_n)
+{
+    const double pi = 3.141592653589793238462643383279502884;
+    tw_m.resize(M);
+    tw_n.resize(M);
+    for (uint32_t t = 0; t < M; t++) {
+        double a = -2.0 * pi * (double)t / (double)M;
+        tw_m[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+        double b = -2.0 * pi * (double)t / (2.0 * (double)M);
+        tw_n[t] = make_float2((float)std::cos(b), (float)std::sin(b));
+    }
+}
+
+// CSR renormalisation per frequency (src/PS/ElectricField.cpp:166-172)
+std::vector<float> csr_renorm(uint32_t nmax, float delta0, double qscale,
+                              float fc, float* fdelta)
+{
+    Ruler fax(nmax, 0.0f, 1.0f / delta0);
+    const float scale_hz = (float)(2.99792458e8 / (double)(float)qscale);
+    const float renorm0 = delta0 * delta0;
+    std::vector<float> r(nmax);
+    for (uint32_t i = 0; i < nmax; i++) {
+        float rn = renorm0;
+        if (fc > 0.0f) {
+            float q = scale_hz * fax.data[i] / fc;
+            double e = 1.0 - std::exp(-std::pow((double)q, 2.0));
+            rn = (float)((double)rn * e);
+        }
+        r[i] = rn;
+    }
+    *fdelta = fax.delta;
+    return r;
+}
+
+FieldParams field_params(vfps_field* f)
+{
+    FieldParams p{};
+    p.nb = f->nb;
+    p.n = f->ps->n;
+    p.M = f->nmax / 2;
+    p.spacing = f->spacing;
+    p.buckets = f->buckets_d;
+    p.scaling = f->wakescaling;
+    p.proj = f->ps->proj;
+    p.ws = f->ps->ws;
+    p.Z = f->z->z;
+    p.tw_m = f->tw_m;
+    p.tw_n = f->tw_n;
+    p.wake = f->wake;
+    p.Zc = f->z->z;
+    p.renorm = f->csr_renorm;
+    p.fdelta = f->fdelta;
+    return p;
+}
+
+int launch_wake_potential(vfps_field* f, float* dst)
+{
+    FieldParams p = field_params(f);
+    p.wake2 = dst;
+    p.padded = f->padded;
+    return launch_field(p, true, false, 1, f->dev->stream);
+}
+
+}  // namespace vfps_impl
+
+using namespace vfps_impl;
+
+extern "C" {
+
+float vfps_field_wakescaling(double Ib, double dt, double qscale,
+                             float delta1, double sigma_delta, double E0)
+{
+    // ps->getScale(0,"Meter") is the float stored in the Ruler's scale map
+    return (float)(Ib * dt * 2.99792458e8 / (double)(float)qscale
+                   / ((double)delta1 * sigma_delta * E0));
+}
+
+int vfps_field_create(vfps_mesh* ps, vfps_impedance* z,
+                      const uint32_t* buckets, uint32_t nbuckets,
+                      uint32_t spacing_bins, double f_rev,
+                      float revolutionpart, float wakescaling,
+                      vfps_field** out)
+{
+    VFPS_CHECK_ARG(ps && z && out && buckets, "ElectricField: NULL argument");
+    VFPS_CHECK_ARG(nbuckets == ps->nb,
+                   "ElectricField: one bucket number per bunch expected");
+    const uint32_t nmax = z->n;
+    VFPS_CHECK_ARG(nmax >= 4 && (nmax & (nmax - 1)) == 0,
+                   "ElectricField: FFT length (impedance size) must be a "
+                   "power of two >= 4");
+    for (uint32_t b = 0; b < nbuckets; b++) {
+        VFPS_CHECK_ARG((size_t)buckets[b] * spacing_bins + ps->n <= nmax,
+                       "ElectricField: bunch does not fit the padded array");
+    }
+    const uint32_t M = nmax / 2;
+    VFPS_CHECK_ARG(field_lds_bytes(M) <= ps->dev->lds_max,
+                   "ElectricField: FFT length exceeds the LDS of one CU");
+    VFPS_TRY_BEGIN
+    int rc = field_set_lds_limit(field_lds_bytes(M));
+    if (rc != VFPS_OK) return rc;
+    auto* f = new vfps_field();
+    f->dev = ps->dev;
+    f->ps = ps;
+    f->z = z;
+    f->nmax = nmax;
+    f->nb = ps->nb;
+    f->spacing = spacing_bins;
+    f->buckets.assign(buckets, buckets + nbuckets);
+    // _wakescaling(wakescaling/_nmax) (src/PS/ElectricField.cpp:58)
+    f->wakescaling = wakescaling / (float)nmax;
+    f->f_rev = f_rev;
+    f->revolutionpart = revolutionpart;
+    std::vector<float2> twm, twn;
+    make_twiddles(M, twm, twn);
+    auto renorm = csr_renorm(nmax, ps->ax0.delta, ps->qscale, 0.0f,
+                             &f->fdelta);
+    f->csr_fc = 0.0f;
+    hipError_t e = hipSuccess;
+    const uint32_t n = ps->n, nb = ps->nb;
+    e = e ? e : hipMalloc(&f->buckets_d, sizeof(uint32_t) * nb);
+    e = e ? e : hipMalloc(&f->wake, sizeof(float) * nb * n);
+    e = e ? e : hipMalloc(&f->padded, sizeof(float) * nmax);
+    e = e ? e : hipMalloc(&f->tw_m, sizeof(float2) * M);
+    e = e ? e : hipMalloc(&f->tw_n, sizeof(float2) * M);
+    e = e ? e : hipMalloc(&f->csr_renorm, sizeof(float) * nmax);
+    e = e ? e : hipMalloc(&f->spectrum, sizeof(float) * nb * nmax);
+    e = e ? e : hipMalloc(&f->power, sizeof(float) * nb);
+    if (e != hipSuccess) {
+        vfps_field_destroy(f);
+        return fail(VFPS_ERR_NOMEM, hipGetErrorString(e));
+    }
+    hipStream_t st = ps->dev->stream;
+    hipMemcpyAsync(f->buckets_d, buckets, sizeof(uint32_t) * nb,
+                   hipMemcpyHostToDevice, st);
+    hipMemsetAsync(f->wake, 0, sizeof(float) * nb * n, st);
+    hipMemsetAsync(f->padded, 0, sizeof(float) * nmax, st);
+    hipMemcpyAsync(f->tw_m, twm.data(), sizeof(float2) * M,
+                   hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(f->tw_n, twn.data(), sizeof(float2) * M,
+                   hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(f->csr_renorm, renorm.data(), sizeof(float) * nmax,
+                   hipMemcpyHostToDevice, st);
+    hipMemsetAsync(f->spectrum, 0, sizeof(float) * nb * nmax, st);
+    hipMemsetAsync(f->power, 0, sizeof(float) * nb, st);
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        vfps_field_destroy(f);
+        return fail(VFPS_ERR_HIP, hipGetErrorString(e));
+    }
+    *out = f;
+    return VFPS_OK;
+    VFPS_TRY_END
+}
+
+int vfps_field_destroy(vfps_field* f)
+{
+    if (!f) return VFPS_OK;
+    hipStreamSynchronize(f->dev->stream);
+    hipFree(f->buckets_d);
+    hipFree(f->wake);
+    hipFree(f->padded);
+    hipFree(f->tw_m);
+    hipFree(f->tw_n);
+    hipFree(f->csr_renorm);
+    hipFree(f->spectrum);
+    hipFree(f->power);
+    delete f;
+    return VFPS_OK;
+}
+
+__global__ void pad_kernel(uint32_t nb, uint32_t n, uint32_t nmax,
+                           uint32_t spacing, const uint32_t* buckets,
+                           const float* proj, float* padded)
+{
+    for (uint32_t i = threadIdx.x; i < nmax; i += blockDim.x) {
+        padded[i] = 0.0f;
+    }
+    __syncthreads();
+    for (uint32_t b = 0; b < nb; b++) {
+        for (uint32_t x = threadIdx.x; x < n; x += blockDim.x) {
+            padded[buckets[b] * spacing + x] = proj[(size_t)b * n + x];
+        }
+    }
+}
+
+int vfps_field_pad(vfps_field* f)
+{
+    VFPS_CHECK_ARG(f, "padBunchProfiles: NULL field");
+    hipLaunchKernelGGL(pad_kernel, dim3(1), dim3(256), 0, f->dev->stream,
+                       f->nb, f->ps->n, f->nmax, f->spacing, f->buckets_d,
+                       f->ps->proj, f->padded);
+    VFPS_HIP(hipGetLastError());
+    return VFPS_OK;
+}
+
+int vfps_field_get_padded(vfps_field* f, float* out)
+{
+    VFPS_CHECK_ARG(f && out, "getPaddedBunchProfiles: NULL argument");
+    VFPS_HIP(hipMemcpyAsync(out, f->padded, sizeof(float) * f->nmax,
+                            hipMemcpyDeviceToHost, f->dev->stream));
+    VFPS_HIP(hipStreamSynchronize(f->dev->stream));
+    return VFPS_OK;
+}
+
+int vfps_field_wake_potential(vfps_field* f, float* wp)
+{
+    VFPS_CHECK_ARG(f, "wakePotential: NULL field");
+    int rc = launch_wake_potential(f, nullptr);
+    if (rc != VFPS_OK) return rc;
+    if (wp) {
+        VFPS_HIP(hipMemcpyAsync(wp, f->wake, sizeof(float) * f->nb * f->ps->n,
+                                hipMemcpyDeviceToHost, f->dev->stream));
+        VFPS_HIP(hipStreamSynchronize(f->dev->stream));
+    }
+    return VFPS_OK;
+}
+
+int vfps_field_get_wakescaling(const vfps_field* f, float* s)
+{
+    VFPS_CHECK_ARG(f && s, "getWakeScaling: NULL argument");
+    *s = f->wakescaling;
+    return VFPS_OK;
+}
+
+int vfps_field_update_csr(vfps_field* f, float fc, float* power,
+                          float* spectrum)
+{
+    VFPS_CHECK_ARG(f, "updateCSR: NULL field");
+    VFPS_TRY_BEGIN
+    if (fc != f->csr_fc) {
+        auto renorm = csr_renorm(f->nmax, f->ps->ax0.delta, f->ps->qscale, fc,
+                                 &f->fdelta);
+        VFPS_HIP(hipMemcpyAsync(f->csr_renorm, renorm.data(),
+                                sizeof(float) * f->nmax,
+                                hipMemcpyHostToDevice, f->dev->stream));
+        VFPS_HIP(hipStreamSynchronize(f->dev->stream));
+        f->csr_fc = fc;
+    }
+    FieldParams p = field_params(f);
+    p.spectrum = f->spectrum;
+    p.power = f->power;
+    int rc = launch_field(p, false, true, f->nb, f->dev->stream);
+    if (rc != VFPS_OK) return rc;
+    if (power) {
+        VFPS_HIP(hipMemcpyAsync(power, f->power, sizeof(float) * f->nb,
+                                hipMemcpyDeviceToHost, f->dev->stream));
+    }
+    if (spectrum) {
+        VFPS_HIP(hipMemcpyAsync(spectrum, f->spectrum,
+                                sizeof(float) * f->nb * f->nmax,
+                                hipMemcpyDeviceToHost, f->dev->stream));
+    }
+    if (power || spectrum) {
+        VFPS_HIP(hipStreamSynchronize(f->dev->stream));
+    }
+    return VFPS_OK;
+    VFPS_TRY_END
+}
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// ElectricField kernel parameters, shared with the fused step.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace vfps_impl {
+
+struct FieldParams {
+    uint32_t nb, n, M, spacing;
+    const uint32_t* buckets;
+    float scaling;            // wake scaling already divided by nmax
+    const float* proj;        // [nb][n] x projection
+    const float* ws;          // Simpson weights (for the FP64 integral)
+    const float2* Z;          // wake impedance [2M]
+    const float2* tw_m;       // e^{-2 pi i t/M}
+    const float2* tw_n;       // e^{-2 pi i k/2M}
+    float* wake;              // [nb][n]
+    float* wake2;             // optional copy (a kick map's offsets)
+    float* padded;            // optional [2M] padded profile
+    double* filling;          // optional [nb+1] FP64 integral (+ total)
+    const float2* Zc;         // CSR impedance [2M]
+    const float* renorm;      // CSR renormalisation [2M]
+    float fdelta;             // frequency step of the CSR sum
+    float* spectrum;          // optional [nb][2M]
+    float* power;             // optional [nb]
+    uint32_t* counter;        // optional step counter (history slot)
+    float* csr_hist;          // optional per-step CSR power history
+    uint32_t hist_cap;        // history capacity (steps)
+};
+
+FieldParams field_params(vfps_field* f);
+size_t field_lds_bytes(uint32_t M);
+int field_set_lds_limit(size_t bytes);
+int launch_field(const FieldParams& p, bool wake, bool csr, uint32_t blocks,
+                 hipStream_t st);
+void make_twiddles(uint32_t M, std::vector<float2>& tw_m,
+                   std::vector<float2>& tw_n);
+std::vector<float> csr_renorm(uint32_t nmax, float delta0, double qscale,
+                              float fc, float* fdelta);
+
+}  // namespace vfps_impl

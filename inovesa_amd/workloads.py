@@ -1,0 +1,112 @@
+"""Synthetic workloads of the benchmark configurations.
+
+The initial state of every workload is generated here, once, as FP32 bytes
+that are handed unchanged to the GPU library and to any checker: a unit
+Gaussian in normalised (z, delta) coordinates (src/PS/PhaseSpace.cpp:471-484
+evaluated on the mesh axes of inc/PS/Ruler.hpp) plus seeded uniform noise in
+[0, 1e-4), renormalised once to unit charge with the Simpson weights of
+src/PS/PhaseSpace.cpp:486-501.
+"""
+import numpy as np
+
+from .vfps import derive
+
+ONE_DIV_ROOT_TWO_PI = np.float32(0.3989422804014327)
+
+
+def ruler(n, vmin, vmax):
+    """inc/PS/Ruler.hpp:31-51 in float32."""
+    vmin, vmax = np.float32(vmin), np.float32(vmax)
+    delta = (vmax - vmin) / np.float32(n - 1)
+    data = vmin + np.arange(n, dtype=np.float32) * delta
+    return data.astype(np.float32), np.float32(delta)
+
+
+def simpson_weights(n, delta0):
+    ws = np.empty(n, np.float32)
+    h03 = np.float32(delta0) / np.float32(3)
+    ws[0] = h03
+    dc = np.float32(1)
+    for x in range(1, n - 1):
+        ws[x] = h03 * (np.float32(3) + dc)
+        dc = -dc
+    ws[n - 1] = h03
+    return ws
+
+
+def initial_mesh(n, seed=1, noise=1e-4, half=6.0):
+    """Gaussian + U[0, noise) on an n x n mesh over +-half; unit charge."""
+    ax, d = ruler(n, -half, half)
+    g = (np.float64(ONE_DIV_ROOT_TWO_PI)
+         * np.exp(-0.5 * ax.astype(np.float64) ** 2)).astype(np.float32)
+    data = g[:, None] * g[None, :]
+    rng = np.random.Generator(np.random.Philox(seed))
+    data = (data + (rng.random((n, n)) * noise).astype(np.float32))
+    data = data.astype(np.float32)
+    ws = simpson_weights(n, d).astype(np.float64)
+    integral = (data.astype(np.float64) @ ws) @ ws
+    data = data * np.float32(1.0 / integral)
+    return np.ascontiguousarray(data, np.float32)
+
+
+def kick_drift_config(n=512, steps_per_ts=2000, gap=0.032, current_factor=1.5,
+                      Ib=None, padding=8.0, damping_time=-1.0):
+    """Plain VFP, kick-drift scheme (src/main.cpp:1034-1041): DriftMap with
+    linear momentum compaction, sinusoidal RFKickMap, parallel-plates CSR
+    shielded by `gap`, FokkerPlanckMap (cubic derivative)."""
+    base = derive(grid_size=n, steps_per_ts=steps_per_ts, vacuum_gap=gap,
+                  padding=padding, damping_time=damping_time)
+    if Ib is None:
+        Ib = current_factor * base["Ith"]
+    d = derive(grid_size=n, steps_per_ts=steps_per_ts, vacuum_gap=gap,
+               padding=padding, damping_time=damping_time, bunch_current=Ib)
+    desc = dict(n=n, qmin=d["qmin"], qmax=d["qmax"], pmin=d["pmin"],
+                pmax=d["pmax"], qscale=d["qscale"], pscale=d["pscale"],
+                scheme=0, interp=4, rot_clamp=0, angle=d["angle"],
+                rf_linear=0, rf_angle=d["angle"],
+                revolutionpart=d["revolutionpart"], V_RF=d["V_eff"],
+                f_RF=d["f_RF"], V0=d["V0"], slip=d["slip"],
+                E0=d["ring"]["beam_energy"], use_fp=1, fptype=3, e1=d["e1"],
+                deriv=4, use_wake=1, spacing_bins=d["spacing_bins"],
+                wakescaling=d["wakescaling"], use_csr=1,
+                fc=d["ring"]["cutoff_frequency"])
+    return desc, d
+
+
+def rotation_config(n=256, steps_per_ts=1000, damping_steps=1000.0,
+                    current_factor=0.5, Ib=None, padding=8.0):
+    """Plain VFP, rotation scheme: WakePotentialMap, RotationMap (cubic,
+    clamped, precomputed semantics), FokkerPlanckMap, free-space CSR."""
+    base = derive(grid_size=n, steps_per_ts=steps_per_ts, vacuum_gap=-1.0,
+                  padding=padding)
+    dt = base["dt"]
+    if Ib is None:
+        Ib = current_factor * base["Ith"]
+    d = derive(grid_size=n, steps_per_ts=steps_per_ts, vacuum_gap=-1.0,
+               padding=padding, damping_time=damping_steps * dt,
+               bunch_current=Ib)
+    desc = dict(n=n, qmin=d["qmin"], qmax=d["qmax"], pmin=d["pmin"],
+                pmax=d["pmax"], qscale=d["qscale"], pscale=d["pscale"],
+                scheme=1, interp=4, rot_clamp=1, angle=d["angle"],
+                rf_linear=1, rf_angle=d["angle"],
+                revolutionpart=d["revolutionpart"], V_RF=d["V_eff"],
+                f_RF=d["f_RF"], V0=d["V0"], slip=d["slip"],
+                E0=d["ring"]["beam_energy"], use_fp=1, fptype=3, e1=d["e1"],
+                deriv=4, use_wake=1, spacing_bins=d["spacing_bins"],
+                wakescaling=d["wakescaling"], use_csr=1,
+                fc=d["ring"]["cutoff_frequency"])
+    return desc, d
+
+
+def impedances(d, dev=None, resistive_wall=None):
+    """Wake and CSR impedance tables of src/main.cpp:717-727 on the device:
+    (wake, csr)."""
+    from .vfps import makeImpedance
+    ring = d["ring"]
+    gap = ring["vacuum_gap"]
+    s, xi = (resistive_wall or (0.0, 0.0))
+    zw = makeImpedance(d["padded_bins"], d["fmax"], d["R_bend"],
+                       ring["f_rev"], gap, True, s, xi, dev=dev)
+    zc = makeImpedance(d["padded_bins"], d["fmax"], d["R_bend"],
+                       ring["f_rev"], gap if gap > 0 else -1.0, dev=dev)
+    return zw, zc

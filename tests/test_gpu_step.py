@@ -1,0 +1,125 @@
+"""The whole time step (src/main.cpp:948-1053) on the GPU."""
+import numpy as np
+import pytest
+
+from _helpers import oracle_cfg, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def V():
+    import inovesa_amd as V
+    return V
+
+
+def _setup(V, scheme, n, **kw):
+    from inovesa_amd.workloads import (impedances, initial_mesh,
+                                       kick_drift_config, rotation_config)
+    if scheme == 0:
+        desc, d = kick_drift_config(n=n, **kw)
+    else:
+        desc, d = rotation_config(n=n, **kw)
+    zw, zc = impedances(d)
+    return desc, d, zw, zc, initial_mesh(n, seed=1)
+
+
+CASES = [(0, 512, {}), (1, 256, {}), (0, 256, {"steps_per_ts": 500}),
+         (1, 128, {"steps_per_ts": 300})]
+
+
+@pytest.mark.parametrize("scheme,n,kw", CASES)
+def test_one_step_matches_oracle(V, orc, scheme, n, kw):
+    desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
+    st = V.Step(desc, init, zw, zc)
+    st.record(4)
+    st.run(1)
+    mesh, proj, wake = st.mesh()
+    csr, _ = st.history()
+    sim = orc.sim(oracle_cfg(desc, zw.n), init, zw.data(), zc.data())
+    ocsr, _ = sim.run(1, want_csr=True)
+    omesh, oproj, owake = sim.get()
+    assert rel_linf(wake, owake) <= 1e-5
+    assert rel_linf(mesh, omesh) <= 1e-5
+    assert rel_linf(proj, oproj) <= 1e-5
+    assert abs(csr[0] - ocsr[0]) <= 1e-5 * abs(ocsr[0])
+
+
+@pytest.mark.parametrize("scheme,n,kw", CASES[:2])
+def test_fused_equals_per_map_path(V, scheme, n, kw):
+    desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
+    a = V.Step(dict(desc, fused=1), init, zw, zc)
+    b = V.Step(dict(desc, fused=0), init, zw, zc)
+    a.record(80, profiles=True)
+    b.record(80, profiles=True)
+    a.run(70)
+    b.run(70)
+    ma, pa, wa = a.mesh()
+    mb, pb, wb = b.mesh()
+    assert np.array_equal(ma, mb)
+    assert np.array_equal(pa, pb)
+    assert np.array_equal(wa, wb)
+    ca, fa = a.history()
+    cb, fb = b.history()
+    assert len(ca) == 70 and np.array_equal(ca, cb) and np.array_equal(fa, fb)
+
+
+def test_graph_replay_deterministic(V):
+    desc, d, zw, zc, init = _setup(V, 0, 256, steps_per_ts=500)
+    a = V.Step(desc, init, zw, zc)
+    b = V.Step(desc, init, zw, zc)
+    a.run(64 * 2 + 3)          # two graph replays and 3 single steps
+    for _ in range(64 * 2 + 3):
+        b.run(1)               # single steps only
+    assert np.array_equal(a.mesh()[0], b.mesh()[0])
+    a.run(64)
+    b.run(64)
+    assert np.array_equal(a.mesh()[0], b.mesh()[0])
+
+
+@pytest.mark.parametrize("tile", [1, 2, 4, 8, 16])
+def test_tile_rows_do_not_change_results(V, tile):
+    desc, d, zw, zc, init = _setup(V, 0, 512)
+    ref = V.Step(dict(desc, fused=0), init, zw, zc)
+    ref.run(3)
+    st = V.Step(dict(desc, tile_rows=tile), init, zw, zc)
+    st.run(3)
+    assert np.array_equal(st.mesh()[0], ref.mesh()[0])
+
+
+@pytest.mark.parametrize("scheme,n,kw", [(0, 256, {"steps_per_ts": 500}),
+                                         (1, 128, {"steps_per_ts": 300})])
+def test_history_and_drift_vs_fp64(V, orc, orc64, scheme, n, kw):
+    steps = 100
+    desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
+    st = V.Step(desc, init, zw, zc)
+    st.record(steps, profiles=True)
+    st.run(steps)
+    csr, prof = st.history()
+    mesh, _, _ = st.mesh()
+    sim = orc.sim(oracle_cfg(desc, zw.n), init, zw.data(), zc.data())
+    ocsr, oprof = sim.run(steps, want_csr=True, want_profiles=True)
+    omesh, _, _ = sim.get()
+    sim64 = orc64.sim(oracle_cfg(desc, zw.n), init.astype(np.float64),
+                      zw.data(), zc.data())
+    sim64.run(steps)
+    m64, _, _ = sim64.get()
+    # per step from the same state the bound is 1e-5; over 100 steps the
+    # FP32 paths drift apart by reordered sums only
+    assert rel_linf(csr[:5], ocsr[:5]) <= 1e-5
+    assert rel_linf(prof[:5], oprof[:5]) <= 1e-5
+    e32 = rel_linf(mesh, omesh)
+    e64 = rel_linf(mesh, m64)
+    o64 = rel_linf(omesh, m64)
+    print(f"\n{steps} steps, scheme {scheme}, n={n}: GPU vs FP32 oracle "
+          f"{e32:.2e}, GPU vs FP64 {e64:.2e}, FP32 oracle vs FP64 {o64:.2e}")
+    assert e32 <= 1e-4
+    assert e64 <= 10 * o64 + 1e-5
+
+
+def test_step_rejects_bad_descriptions(V):
+    desc, d, zw, zc, init = _setup(V, 0, 128, steps_per_ts=300)
+    with pytest.raises(V.VfpsInvalid):
+        V.Step(dict(desc, scheme=3), init, zw, zc)
+    with pytest.raises(V.VfpsInvalid):
+        V.Step(desc, init, None, zc)
