@@ -314,6 +314,8 @@ typedef struct {
     float fc;
     int32_t fused;            /* 1: fused kernels (default), 0: one kernel per map */
     uint32_t tile_rows;       /* 0: autotuned; rows per workgroup of the fused kernel */
+    int32_t fast_projection;  /* 0: x projection summed in the reference's
+                                 order (bit-exact); 1: wavefront tree sum */
 } vfps_step_desc;
 
 VFPS_API int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,

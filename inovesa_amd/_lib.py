@@ -77,7 +77,7 @@ class StepDesc(C.Structure):
                 ("e1", _f32), ("deriv", _u32), ("use_wake", _i32),
                 ("spacing_bins", _u32), ("wakescaling", _f32),
                 ("use_csr", _i32), ("fc", _f32), ("fused", _i32),
-                ("tile_rows", _u32)]
+                ("tile_rows", _u32), ("fast_projection", _i32)]
 
 
 _SIGS = {

@@ -132,8 +132,8 @@ struct vfps_field {
     float revolutionpart = 1;
     float* wake = nullptr;       // [nb][n]
     float* padded = nullptr;     // [nmax]
-    float2* tw_m = nullptr;      // e^{-2 pi i t/M}, t < M (M = nmax/2)
-    float2* tw_n = nullptr;      // e^{-2 pi i k/nmax}, k < M
+    double2* tw_m = nullptr;     // e^{-2 pi i t/M}, t < M (M = nmax/2)
+    double2* tw_n = nullptr;     // e^{-2 pi i k/nmax}, k < M
     float* csr_renorm = nullptr; // [nmax]
     float* spectrum = nullptr;   // [nb][nmax]
     float* power = nullptr;      // [nb]

@@ -18,11 +18,11 @@ template <bool WAKE, bool CSR>
 __global__ void __launch_bounds__(512)
 field_kernel(FieldParams p)
 {
-    extern __shared__ float2 lds[];
+    extern __shared__ double2 lds[];
     field_body<WAKE, CSR>(p, blockIdx.x, lds);
 }
 
-size_t field_lds_bytes(uint32_t M) { return sizeof(float2) * 2 * (size_t)M; }
+size_t field_lds_bytes(uint32_t M) { return sizeof(double2) * 2 * (size_t)M; }
 
 int launch_field(const FieldParams& p, bool wake, bool csr, uint32_t blocks,
                  hipStream_t st)
@@ -60,18 +60,18 @@ int field_set_lds_limit(size_t bytes)
 }
 
 // twiddle tables: e^{-2 pi i t/M} (t < M) and e^{-2 pi i k/N} (k < M),
-// computed in double, rounded to float
-void make_twiddles(uint32_t M, std::vector<float2>& tw_m,
-                   std::vector<float2>& tw_n)
+// in double
+void make_twiddles(uint32_t M, std::vector<double2>& tw_m,
+                   std::vector<double2>& tw_n)
 {
     const double pi = 3.141592653589793238462643383279502884;
     tw_m.resize(M);
     tw_n.resize(M);
     for (uint32_t t = 0; t < M; t++) {
         double a = -2.0 * pi * (double)t / (double)M;
-        tw_m[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+        tw_m[t] = make_double2(std::cos(a), std::sin(a));
         double b = -2.0 * pi * (double)t / (2.0 * (double)M);
-        tw_n[t] = make_float2((float)std::cos(b), (float)std::sin(b));
+        tw_n[t] = make_double2(std::cos(b), std::sin(b));
     }
 }
 
@@ -174,7 +174,7 @@ int vfps_field_create(vfps_mesh* ps, vfps_impedance* z,
     f->wakescaling = wakescaling / (float)nmax;
     f->f_rev = f_rev;
     f->revolutionpart = revolutionpart;
-    std::vector<float2> twm, twn;
+    std::vector<double2> twm, twn;
     make_twiddles(M, twm, twn);
     auto renorm = csr_renorm(nmax, ps->ax0.delta, ps->qscale, 0.0f,
                              &f->fdelta);
@@ -184,8 +184,8 @@ int vfps_field_create(vfps_mesh* ps, vfps_impedance* z,
     e = e ? e : hipMalloc(&f->buckets_d, sizeof(uint32_t) * nb);
     e = e ? e : hipMalloc(&f->wake, sizeof(float) * nb * n);
     e = e ? e : hipMalloc(&f->padded, sizeof(float) * nmax);
-    e = e ? e : hipMalloc(&f->tw_m, sizeof(float2) * M);
-    e = e ? e : hipMalloc(&f->tw_n, sizeof(float2) * M);
+    e = e ? e : hipMalloc(&f->tw_m, sizeof(double2) * M);
+    e = e ? e : hipMalloc(&f->tw_n, sizeof(double2) * M);
     e = e ? e : hipMalloc(&f->csr_renorm, sizeof(float) * nmax);
     e = e ? e : hipMalloc(&f->spectrum, sizeof(float) * nb * nmax);
     e = e ? e : hipMalloc(&f->power, sizeof(float) * nb);
@@ -198,9 +198,9 @@ int vfps_field_create(vfps_mesh* ps, vfps_impedance* z,
                    hipMemcpyHostToDevice, st);
     hipMemsetAsync(f->wake, 0, sizeof(float) * nb * n, st);
     hipMemsetAsync(f->padded, 0, sizeof(float) * nmax, st);
-    hipMemcpyAsync(f->tw_m, twm.data(), sizeof(float2) * M,
+    hipMemcpyAsync(f->tw_m, twm.data(), sizeof(double2) * M,
                    hipMemcpyHostToDevice, st);
-    hipMemcpyAsync(f->tw_n, twn.data(), sizeof(float2) * M,
+    hipMemcpyAsync(f->tw_n, twn.data(), sizeof(double2) * M,
                    hipMemcpyHostToDevice, st);
     hipMemcpyAsync(f->csr_renorm, renorm.data(), sizeof(float) * nmax,
                    hipMemcpyHostToDevice, st);

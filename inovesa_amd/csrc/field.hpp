@@ -14,8 +14,8 @@ struct FieldParams {
     const float* proj;        // [nb][n] x projection
     const float* ws;          // Simpson weights (for the FP64 integral)
     const float2* Z;          // wake impedance [2M]
-    const float2* tw_m;       // e^{-2 pi i t/M}
-    const float2* tw_n;       // e^{-2 pi i k/2M}
+    const double2* tw_m;      // e^{-2 pi i t/M}
+    const double2* tw_n;      // e^{-2 pi i k/2M}
     float* wake;              // [nb][n]
     float* wake2;             // optional copy (a kick map's offsets)
     float* padded;            // optional [2M] padded profile
@@ -35,8 +35,8 @@ size_t field_lds_bytes(uint32_t M);
 int field_set_lds_limit(size_t bytes);
 int launch_field(const FieldParams& p, bool wake, bool csr, uint32_t blocks,
                  hipStream_t st);
-void make_twiddles(uint32_t M, std::vector<float2>& tw_m,
-                   std::vector<float2>& tw_n);
+void make_twiddles(uint32_t M, std::vector<double2>& tw_m,
+                   std::vector<double2>& tw_n);
 std::vector<float> csr_renorm(uint32_t nmax, float delta0, double qscale,
                               float fc, float* fdelta);
 

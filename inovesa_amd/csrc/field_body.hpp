@@ -8,21 +8,26 @@
 
 namespace vfps_impl {
 
-// WAKE: wake potential of all bunches (profiles at bucket*spacing).
-// CSR: CSR spectrum/power; alone it handles bunch blockIdx.x with its
-// profile at offset 0 (src/PS/ElectricField.cpp:152-163); fused with WAKE
+// WAKE: wake potential of all bunches (profiles at bucket*spacing,
+// src/PS/ElectricField.cpp:186-263).
+// CSR: CSR spectrum/power (src/PS/ElectricField.cpp:133-184); alone it
+// handles bunch `block_bunch` with its profile at offset 0; fused with WAKE
 // (one bunch at offset 0) it reuses the wake's forward transform.
+// The transforms run in FP64 (fft.hpp); the spectrum F, the product
+// W = Z F and the wake potential are rounded to FP32 where the reference
+// holds them as float, and the float arithmetic on them follows the
+// reference.  lds: 2*M double2.
 template <bool WAKE, bool CSR>
 __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
-                           float2* lds)
+                           double2* lds)
 {
-    __shared__ float red[16];
+    __shared__ double red[16];
     const uint32_t M = p.M;
-    float2* A = lds;
-    float2* B = lds + M;
-    float* Af = reinterpret_cast<float*>(A);
+    double2* A = lds;
+    double2* B = lds + M;
+    double* Ad = reinterpret_cast<double*>(A);
     for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
-        A[i] = make_float2(0.0f, 0.0f);
+        A[i] = make_double2(0.0, 0.0);
     }
     __syncthreads();
     // padBunchProfiles (src/PS/ElectricField.cpp:255-263)
@@ -30,19 +35,19 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
         for (uint32_t b = 0; b < p.nb; b++) {
             const uint32_t base = p.buckets[b] * p.spacing;
             for (uint32_t x = threadIdx.x; x < p.n; x += blockDim.x) {
-                Af[base + x] = p.proj[(size_t)b * p.n + x];
+                Ad[base + x] = (double)p.proj[(size_t)b * p.n + x];
             }
         }
     } else {
         const uint32_t b = block_bunch;
         for (uint32_t x = threadIdx.x; x < p.n; x += blockDim.x) {
-            Af[x] = p.proj[(size_t)b * p.n + x];
+            Ad[x] = (double)p.proj[(size_t)b * p.n + x];
         }
     }
     __syncthreads();
     if (p.padded) {
         for (uint32_t i = threadIdx.x; i < 2 * M; i += blockDim.x) {
-            p.padded[i] = Af[i];
+            p.padded[i] = (float)Ad[i];
         }
     }
     if (p.filling) {
@@ -71,58 +76,61 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
         }
         return;
     }
-    float2* Zf = fft_lds<false>(A, B, M, p.tw_m);
-    float2* other = (Zf == A) ? B : A;
-    float csr = 0.0f;
+    double2* Zf = fft_lds<false>(A, B, M, p.tw_m);
+    double2* other = (Zf == A) ? B : A;
+    // CSR power: each bin's float term as the reference forms it, summed
+    // in FP64 (the reference's sequential float sum carries a rounding
+    // error of up to nmax*eps; the FP64 sum is the exact sum of the terms)
+    double csr = 0.0;
     const size_t sbase = CSR ? (size_t)(WAKE ? 0 : block_bunch) * 2 * M : 0;
+    // CSR spectrum term of one bin from the float spectrum value
+    auto csr_bin = [&](uint32_t k, float2 f) {
+        const float a = hypotf(f.x, f.y);  // std::norm via std::abs
+        const float s = p.renorm[k] * p.Zc[k].x * (a * a);
+        if (p.spectrum) {
+            p.spectrum[sbase + k] = s;
+        }
+        return (double)(p.fdelta * s);
+    };
     for (uint32_t k = threadIdx.x; k <= M / 2; k += blockDim.x) {
         if (k == 0) {
-            const float2 z0 = Zf[0];
-            const float f0 = z0.x + z0.y;  // F_0 (real)
-            const float fm = z0.x - z0.y;  // F_M (real)
+            const double2 z0 = Zf[0];
+            const float f0 = (float)(z0.x + z0.y);  // F_0 (real)
+            const float fm = (float)(z0.x - z0.y);  // F_M (real)
             if (CSR) {
-                const float a0 = fabsf(f0), am = fabsf(fm);
-                const float s0 = p.renorm[0] * p.Zc[0].x * (a0 * a0);
-                const float sm = p.renorm[M] * p.Zc[M].x * (am * am);
-                if (p.spectrum) {
-                    p.spectrum[sbase] = s0;
-                    p.spectrum[sbase + M] = sm;
-                }
-                csr += p.fdelta * s0 + p.fdelta * sm;
+                csr += csr_bin(0, make_float2(f0, 0.0f));
+                csr += csr_bin(M, make_float2(fm, 0.0f));
             }
             if (WAKE) {
-                // W_0 = Z_0 F_0 with its imaginary part dropped (c2r
-                // ignores it); W_M is never assigned by the reference and
-                // is taken as zero.
-                const float w0 = p.Z[0].x * f0;
-                Zf[0] = make_float2(w0, w0);
+                // W_0 = Z_0 F_0; c2r ignores its imaginary part.  W_M is
+                // never assigned by the reference and is taken as zero.
+                const float2 w0 = cmulf(p.Z[0], make_float2(f0, 0.0f));
+                Zf[0] = make_double2((double)w0.x, (double)w0.x);
             }
         } else {
             const uint32_t kp = M - k;
-            float2 fk, fkp;
+            double2 fk, fkp;
             r2c_pair(Zf[k], Zf[kp], p.tw_n[k], &fk, &fkp);
+            const float2 f1 = make_float2((float)fk.x, (float)fk.y);
+            const float2 f2 = make_float2((float)fkp.x, (float)fkp.y);
             if (CSR) {
-                const float ak = hypotf(fk.x, fk.y);
-                const float sk = p.renorm[k] * p.Zc[k].x * (ak * ak);
-                csr += p.fdelta * sk;
-                if (p.spectrum) p.spectrum[sbase + k] = sk;
+                csr += csr_bin(k, f1);
                 if (kp != k) {
-                    const float akp = hypotf(fkp.x, fkp.y);
-                    const float skp = p.renorm[kp] * p.Zc[kp].x * (akp * akp);
-                    csr += p.fdelta * skp;
-                    if (p.spectrum) p.spectrum[sbase + kp] = skp;
+                    csr += csr_bin(kp, f2);
                 }
             }
             if (WAKE) {
-                const float2 wk = cmulf(p.Z[k], fk);
-                const float2 wkp = cmulf(p.Z[kp], fkp);
-                float2 yk, ykp;
+                const float2 w1 = cmulf(p.Z[k], f1);
+                const float2 w2 = cmulf(p.Z[kp], f2);
+                const double2 wk = make_double2(w1.x, w1.y);
+                const double2 wkp = make_double2(w2.x, w2.y);
                 if (kp != k) {
+                    double2 yk, ykp;
                     c2r_pair(wk, wkp, p.tw_n[k], &yk, &ykp);
                     Zf[k] = yk;
                     Zf[kp] = ykp;
                 } else {
-                    Zf[k] = make_float2(2.0f * wk.x, -2.0f * wk.y);
+                    Zf[k] = make_double2(2.0 * wk.x, -2.0 * wk.y);
                 }
             }
         }
@@ -135,14 +143,14 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
         }
         csr = block_sum(csr, red);
         if (threadIdx.x == 0 && p.power) {
-            p.power[WAKE ? 0 : block_bunch] = csr;
+            p.power[WAKE ? 0 : block_bunch] = (float)csr;
         }
     }
     if (threadIdx.x == 0 && p.counter) {
         // per-step history: this step's CSR power, then count the step
         const uint32_t c = *p.counter;
         if (CSR && p.csr_hist && c < p.hist_cap) {
-            p.csr_hist[c] = csr;
+            p.csr_hist[c] = (float)csr;
         }
         *p.counter = c + 1;
     }
@@ -150,12 +158,12 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
         return;
     }
     __syncthreads();
-    float2* y = fft_lds<true>(Zf, other, M, p.tw_m);
-    const float* yf = reinterpret_cast<const float*>(y);
+    double2* y = fft_lds<true>(Zf, other, M, p.tw_m);
+    const double* yd = reinterpret_cast<const double*>(y);
     for (uint32_t b = 0; b < p.nb; b++) {
         const uint32_t base = p.buckets[b] * p.spacing;
         for (uint32_t x = threadIdx.x; x < p.n; x += blockDim.x) {
-            const float v = p.scaling * yf[base + x];
+            const float v = p.scaling * (float)yd[base + x];
             p.wake[(size_t)b * p.n + x] = v;
             if (p.wake2) {
                 p.wake2[(size_t)b * p.n + x] = v;
@@ -163,6 +171,5 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
         }
     }
 }
-
 
 }  // namespace vfps_impl

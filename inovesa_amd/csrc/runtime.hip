@@ -85,24 +85,23 @@ __global__ void outer_kernel(uint32_t nb, uint32_t n, const float* g,
     data[i] = g[x] * g[n + y];
 }
 
-// updateXProjection (src/PS/PhaseSpace.cpp:291-323): one wavefront per row
+// updateXProjection (src/PS/PhaseSpace.cpp:291-323): one thread per row,
+// summing y = 0..n-1 in order like std::inner_product, so the projection
+// equals the reference's bit for bit (the wake potential amplifies the
+// last bits of the profile through the high-frequency impedance).
 __global__ void project_x_kernel(uint32_t rows, uint32_t n, const float* data,
                                  const float* ws, float* proj)
 {
-    const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const uint32_t row = blockIdx.x * (blockDim.x / 64) + wave;
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= rows) {
         return;
     }
     const float* r = data + (size_t)row * n;
     float acc = 0.0f;
-    for (uint32_t y = lane; y < n; y += 64) {
+    for (uint32_t y = 0; y < n; y++) {
         acc = acc + r[y] * ws[y];
     }
-    acc = wave_sum(acc);
-    if (lane == 0) {
-        proj[row] = acc;
-    }
+    proj[row] = acc;
 }
 
 // updateYProjection (src/PS/PhaseSpace.cpp:325-340): one thread per column
@@ -200,7 +199,7 @@ __global__ void variance_kernel(uint32_t n, const float* proj,
 int launch_project_x(vfps_mesh* m, const float* src, float* proj)
 {
     const uint32_t rows = m->nb * m->n;
-    hipLaunchKernelGGL(project_x_kernel, dim3(ceil_div(rows, 4)), dim3(256), 0,
+    hipLaunchKernelGGL(project_x_kernel, dim3(ceil_div(rows, 64)), dim3(64), 0,
                        m->dev->stream, rows, m->n, src, m->ws, proj);
     VFPS_HIP(hipGetLastError());
     return VFPS_OK;

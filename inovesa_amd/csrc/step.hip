@@ -211,6 +211,10 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
             slot = c - 1;
         }
     }
+    // rows of the output tile, padded to n+1 floats so that one thread per
+    // row reads without bank conflicts
+    float* tile_out = lds + (size_t)2 * stride * n;
+    const bool seq = d.seq_proj != 0;
     for (int orow = (int)wave; orow < rows_out; orow += (int)nw) {
         const uint32_t x = (uint32_t)(x0 + orow);
         const float* src = oth + (size_t)orow * n;
@@ -228,10 +232,33 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
                 }
             }
             dst[y] = v;
-            acc = acc + v * d.ws[y];
+            if (seq) {
+                tile_out[(size_t)orow * (n + 1) + y] = v;
+            } else {
+                acc = acc + v * d.ws[y];
+            }
         }
-        acc = wave_sum(acc);
-        if (lane == 0) {
+        if (!seq) {
+            acc = wave_sum(acc);
+            if (lane == 0) {
+                d.proj[x] = acc;
+                if (slot != 0xFFFFFFFFu) {
+                    d.prof_hist[(size_t)slot * n + x] = acc;
+                }
+            }
+        }
+    }
+    if (seq) {
+        // the reference's std::inner_product order: y = 0..n-1, one thread
+        // per row
+        __syncthreads();
+        if ((int)tid < rows_out) {
+            const float* r = tile_out + (size_t)tid * (n + 1);
+            float acc = 0.0f;
+            for (uint32_t y = 0; y < n; y++) {
+                acc = acc + r[y] * d.ws[y];
+            }
+            const uint32_t x = (uint32_t)x0 + tid;
             d.proj[x] = acc;
             if (slot != 0xFFFFFFFFu) {
                 d.prof_hist[(size_t)slot * n + x] = acc;
@@ -281,7 +308,7 @@ static const void* maps_kernel_for(const MapsVariant& v)
 
 size_t fused_lds_bytes(uint32_t n, uint32_t T, uint32_t hmax)
 {
-    return sizeof(float) * 2 * (size_t)(T + 2 * hmax) * n;
+    return sizeof(float) * (2 * (size_t)(T + 2 * hmax) * n + (size_t)T * (n + 1));
 }
 
 int fused_set_lds_limit(const MapsVariant& v, size_t bytes)
@@ -322,7 +349,7 @@ template <bool WAKE, bool CSR>
 __global__ void __launch_bounds__(512)
 fields_kernel(const FieldParams* __restrict__ params)
 {
-    extern __shared__ float2 flds[];
+    extern __shared__ double2 flds[];
     const FieldParams p = params[blockIdx.x];
     field_body<WAKE, CSR>(p, 0, flds);
 }
@@ -428,6 +455,7 @@ struct vfps_step {
     vfps_impedance* zwake = nullptr;
     vfps_impedance* zcsr = nullptr;
     float* csr_renorm = nullptr;
+    double2* csr_tw = nullptr;
     float csr_fdelta = 0;
     double* filling = nullptr;  // [2]
     uint32_t* counter = nullptr;
@@ -507,6 +535,7 @@ static void step_free(vfps_step* s)
     vfps_mesh_destroy(s->t2);
     vfps_mesh_destroy(s->t3);
     hipFree(s->csr_renorm);
+    hipFree(s->csr_tw);
     hipFree(s->filling);
     hipFree(s->counter);
     hipFree(s->err);
@@ -537,6 +566,7 @@ static int step_upload_desc(vfps_step* s)
     d.hist_cap = s->hist_cap;
     d.counter = s->counter;
     d.err = s->err;
+    d.seq_proj = s->desc.fast_projection ? 0u : 1u;
     uint32_t hlo = 0, hhi = 0;
     if (s->desc.scheme == 0) {
         std::vector<float> off(t1->n);
@@ -577,8 +607,8 @@ static int step_upload_desc(vfps_step* s)
         p.fdelta = s->csr_fdelta;
         if (!s->wfield) {
             // CSR only: twiddles from a field-less table
-            p.tw_m = reinterpret_cast<const float2*>(s->csr_renorm + 2 * s->M);
-            p.tw_n = p.tw_m + s->M;
+            p.tw_m = s->csr_tw;
+            p.tw_n = s->csr_tw + s->M;
         }
     }
     VFPS_HIP(hipMemcpy(s->fparams_d, &p, sizeof(FieldParams),
@@ -699,18 +729,17 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
         const uint32_t nmax = zcsr->n;
         auto renorm = csr_renorm(nmax, s->t1->ax0.delta, desc->qscale, desc->fc,
                                  &s->csr_fdelta);
-        std::vector<float2> twm, twn;
+        std::vector<double2> twm, twn;
         make_twiddles(s->M, twm, twn);
-        // renorm table followed by twiddles (used when there is no wake field)
-        VFPS_HIP(hipMalloc(&s->csr_renorm, sizeof(float) * nmax
-                                               + sizeof(float2) * 2 * s->M));
+        // twiddles of their own, used when there is no wake field
+        VFPS_HIP(hipMalloc(&s->csr_renorm, sizeof(float) * nmax));
         VFPS_HIP(hipMemcpy(s->csr_renorm, renorm.data(), sizeof(float) * nmax,
                            hipMemcpyHostToDevice));
-        float2* tw = reinterpret_cast<float2*>(s->csr_renorm + nmax);
-        VFPS_HIP(hipMemcpy(tw, twm.data(), sizeof(float2) * s->M,
+        VFPS_HIP(hipMalloc(&s->csr_tw, sizeof(double2) * 2 * s->M));
+        VFPS_HIP(hipMemcpy(s->csr_tw, twm.data(), sizeof(double2) * s->M,
                            hipMemcpyHostToDevice));
-        VFPS_HIP(hipMemcpy(tw + s->M, twn.data(), sizeof(float2) * s->M,
-                           hipMemcpyHostToDevice));
+        VFPS_HIP(hipMemcpy(s->csr_tw + s->M, twn.data(),
+                           sizeof(double2) * s->M, hipMemcpyHostToDevice));
     }
     VFPS_HIP(hipMalloc(&s->filling, sizeof(double) * 2));
     VFPS_HIP(hipMalloc(&s->counter, sizeof(uint32_t)));

@@ -27,6 +27,7 @@ struct InstDesc {
     uint32_t hist_cap;
     const uint32_t* counter;      // steps counted by the field kernel
     uint32_t* err;                // set if a tap falls outside the halo
+    uint32_t seq_proj;            // 1: x projection in the reference's order
 };
 
 // kernel variant key: scheme (0 kick-drift, 1 rotation), interpolation,

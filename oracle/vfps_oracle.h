@@ -155,7 +155,8 @@ int or_fp_build(uint32_t n, uint32_t fptype, real e1, uint32_t deriv,
 void or_fp_apply(uint32_t nb, uint32_t n, uint32_t ip, const or_hi* hinfo,
                  const real* in, real* out);
 
-/* ---- FFT (stands in for FFTW's fftwf_plan_dft_r2c_1d / c2r_1d) -------- *
+/* ---- FFT (stands in for FFTW's fftwf_plan_dft_r2c_1d / c2r_1d; computed in
+ * double and rounded to real, the correctly rounded FP32 transform) ------ *
  * r2c: out[k] = sum_x in[x] exp(-2 pi i k x / n), k = 0..n/2.
  * c2r: out[x] = Re W0 + 2 sum_{k=1}^{n/2-1} Re(W_k e^{+2pi i k x/n})
  *               + Re W_{n/2} (-1)^x     (unnormalised, imag of DC and

@@ -172,4 +172,11 @@ def test_csr_vs_oracle(V, orc, n, fc):
     rspec, rpower = orc.update_csr(prof, z.data(), ps.getDelta(0),
                                    np.float32(d["qscale"]), np.float32(fc))
     assert rel_linf(spec, rspec) <= 1e-5
-    assert abs(power[0] - rpower[0]) <= 1e-5 * abs(rpower[0])
+    # the power is the FP64 sum of the reference's float terms; the
+    # reference sums them sequentially in float (error <= nmax*eps*sum)
+    nmax = spec.shape[1]
+    fdelta = np.float32(np.float32(1) / ps.getDelta(0)) / np.float32(nmax - 1)
+    terms = (fdelta * rspec[0]).astype(np.float64)
+    exact = terms.sum()
+    assert abs(power[0] - exact) <= 1e-6 * abs(exact)
+    assert abs(rpower[0] - exact) <= nmax * 2.0 ** -23 * np.abs(terms).sum()

@@ -42,7 +42,25 @@ def test_one_step_matches_oracle(V, orc, scheme, n, kw):
     assert rel_linf(wake, owake) <= 1e-5
     assert rel_linf(mesh, omesh) <= 1e-5
     assert rel_linf(proj, oproj) <= 1e-5
-    assert abs(csr[0] - ocsr[0]) <= 1e-5 * abs(ocsr[0])
+    _check_csr_power(orc, desc, zc, init, csr[0], ocsr[0])
+
+
+def _check_csr_power(orc, desc, zc, init, gpu_power, ref_power):
+    """The GPU sums the reference's float CSR terms in FP64; the reference
+    sums them sequentially in float (error <= nmax*eps*sum)."""
+    import math
+    ax, dq, _ = orc.ruler(desc["n"], desc["qmin"], desc["qmax"])
+    ws = orc.simpson_weights(desc["n"], dq)
+    proj = orc.project_x(init.reshape(1, desc["n"], desc["n"]), ws)
+    spec, _ = orc.update_csr(proj, zc.data(), dq, np.float32(desc["qscale"]),
+                             np.float32(desc["fc"]))
+    nmax = spec.shape[1]
+    fdelta = np.float32(np.float32(1) / dq) / np.float32(nmax - 1)
+    terms = (fdelta * spec[0]).astype(np.float64)
+    exact = terms.sum()
+    assert abs(gpu_power - exact) <= 1e-6 * abs(exact)
+    gamma = nmax * 2.0 ** -24 / (1 - nmax * 2.0 ** -24)
+    assert abs(ref_power - exact) <= gamma * np.abs(terms).sum()
 
 
 @pytest.mark.parametrize("scheme,n,kw", CASES[:2])
@@ -62,6 +80,20 @@ def test_fused_equals_per_map_path(V, scheme, n, kw):
     ca, fa = a.history()
     cb, fb = b.history()
     assert len(ca) == 70 and np.array_equal(ca, cb) and np.array_equal(fa, fb)
+
+
+def test_fast_projection_mode(V, orc):
+    desc, d, zw, zc, init = _setup(V, 0, 512)
+    st = V.Step(dict(desc, fast_projection=1), init, zw, zc)
+    st.run(1)
+    mesh, proj, wake = st.mesh()
+    sim = orc.sim(oracle_cfg(desc, zw.n), init, zw.data(), zc.data())
+    sim.run(1)
+    omesh, oproj, owake = sim.get()
+    # the tree-summed profile differs in its last bits; the mesh stays close
+    assert rel_linf(proj, oproj) <= 1e-5
+    assert rel_linf(mesh, omesh) <= 1e-5
+    assert rel_linf(wake, owake) <= 1e-4
 
 
 def test_graph_replay_deterministic(V):
@@ -106,7 +138,7 @@ def test_history_and_drift_vs_fp64(V, orc, orc64, scheme, n, kw):
     m64, _, _ = sim64.get()
     # per step from the same state the bound is 1e-5; over 100 steps the
     # FP32 paths drift apart by reordered sums only
-    assert rel_linf(csr[:5], ocsr[:5]) <= 1e-5
+    assert rel_linf(csr[:5], ocsr[:5]) <= 2e-4  # float sequential sum (ref)
     assert rel_linf(prof[:5], oprof[:5]) <= 1e-5
     e32 = rel_linf(mesh, omesh)
     e64 = rel_linf(mesh, m64)
