@@ -340,6 +340,12 @@ VFPS_API int vfps_step_set_mesh(vfps_step* s, const float* mesh);
  * "wake", "maps" */
 VFPS_API int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
                               float* ms_maps, float* ms_step);
+/* one fused step with device timestamps at the phase boundaries of the
+ * first workgroup of each kernel; ns_out (>= 16): [1..6] maps kernel
+ * (loads, wake kick, RF kick, drift/rotation, FP, projection) and [9..13]
+ * field kernel (loads, forward FFT, spectrum pass, inverse FFT, output),
+ * nanoseconds since the kernel's first stamp; [15] the clock period */
+VFPS_API int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout);
 
 #ifdef __cplusplus
 }

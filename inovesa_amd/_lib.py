@@ -154,6 +154,7 @@ _SIGS = {
     "vfps_step_get_mesh": (_int, [_vp, _vp, _vp, _vp]),
     "vfps_step_set_mesh": (_int, [_vp, _vp]),
     "vfps_step_timing": (_int, [_vp, _u32, _vp, _vp, _vp]),
+    "vfps_step_phases": (_int, [_vp, _vp, _u32]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

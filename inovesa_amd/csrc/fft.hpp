@@ -3,8 +3,8 @@
 // A real sequence of length N = 2M is transformed through one complex FFT
 // of length M (even samples in the real parts, odd samples in the
 // imaginary parts).  The complex FFT is a Stockham autosort with radix-4
-// stages and one radix-2 stage when log2(M) is odd; it ping-pongs between
-// two LDS buffers and needs no bit reversal.
+// stages and one radix-2 stage when log2(M) is odd, in place in one LDS
+// buffer, with no bit reversal.
 //
 // The butterflies run in FP64.  Inputs and outputs stay FP32 exactly where
 // the reference's arrays are float (profile, spectrum F, W = Z F, wake
@@ -30,30 +30,67 @@ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
-// Complex FFT of length M (power of two, >= 2) held in LDS buffer `a`;
-// `b` is scratch of the same size.  tw[t] = exp(-2 pi i t / M), t < M.
-// Forward uses exp(-i..), INV uses exp(+i..); unnormalised.  Returns the
-// buffer holding the result.  Must be called by the whole workgroup.
-template <bool INV>
-__device__ double2* fft_lds(double2* a, double2* b, uint32_t M,
-                            const double2* __restrict__ tw)
+// twiddle exp(-2 pi i t / M) for t < M from the half table
+// tw[t] (t < M/2), using exp(-2 pi i (t + M/2) / M) = -exp(-2 pi i t / M)
+__device__ __forceinline__ double2 twiddle(const double2* tw, uint32_t t,
+                                           uint32_t half)
 {
+    if (t < half) {
+        return tw[t];
+    }
+    const double2 w = tw[t - half];
+    return make_double2(-w.x, -w.y);
+}
+
+// Largest number of butterflies one thread handles per stage (the field
+// kernels run 1024 threads: M <= 8192)
+constexpr int FFT_MAX_BF = 2;
+constexpr uint32_t FFT_THREADS = 1024;
+
+// Complex FFT of length M (power of two, 2 <= M <= 4 * FFT_MAX_BF *
+// blockDim.x) in place in the LDS buffer `a`.  Each stage reads all its
+// inputs into registers, synchronises, then writes the outputs to their
+// Stockham positions, so one buffer suffices.  tw: LDS half table
+// exp(-2 pi i t / M), t < M/2.  Forward uses exp(-i..), INV exp(+i..);
+// unnormalised.  Must be called by the whole workgroup.
+template <bool INV>
+__device__ void fft_lds(double2* a, uint32_t M, const double2* tw)
+{
+    const uint32_t half = M / 2;
     uint32_t Ns = 1;
     while (Ns < M) {
         const uint32_t R = ((M / Ns) % 4u == 0u) ? 4u : 2u;
         const uint32_t nbf = M / R;
         const uint32_t tscale = M / (Ns * R);
-        for (uint32_t j = threadIdx.x; j < nbf; j += blockDim.x) {
+        double2 v[FFT_MAX_BF][4];
+#pragma unroll
+        for (int b = 0; b < FFT_MAX_BF; b++) {
+            const uint32_t j = threadIdx.x + b * blockDim.x;
+            if (j < nbf) {
+                v[b][0] = a[j];
+                v[b][1] = a[j + nbf];
+                if (R == 4) {
+                    v[b][2] = a[j + 2 * nbf];
+                    v[b][3] = a[j + 3 * nbf];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < FFT_MAX_BF; b++) {
+            const uint32_t j = threadIdx.x + b * blockDim.x;
+            if (j >= nbf) {
+                continue;
+            }
             const uint32_t jm = j & (Ns - 1);
             const uint32_t tstep = jm * tscale;
             const uint32_t idxD = (j - jm) * R + jm;
             if (R == 4) {
-                double2 v0 = a[j];
-                double2 v1 = a[j + nbf];
-                double2 v2 = a[j + 2 * nbf];
-                double2 v3 = a[j + 3 * nbf];
+                double2 v0 = v[b][0], v1 = v[b][1], v2 = v[b][2], v3 = v[b][3];
                 if (tstep != 0) {
-                    double2 w1 = tw[tstep], w2 = tw[2 * tstep], w3 = tw[3 * tstep];
+                    double2 w1 = twiddle(tw, tstep, half);
+                    double2 w2 = twiddle(tw, 2 * tstep, half);
+                    double2 w3 = twiddle(tw, 3 * tstep, half);
                     if (INV) {
                         w1 = cconj(w1);
                         w2 = cconj(w2);
@@ -67,31 +104,26 @@ __device__ double2* fft_lds(double2* a, double2* b, uint32_t M,
                 const double2 a2 = cadd(v1, v3), d = csub(v1, v3);
                 // forward: -i d; inverse: +i d
                 const double2 a3 = INV ? make_double2(-d.y, d.x) : make_double2(d.y, -d.x);
-                b[idxD] = cadd(a0, a2);
-                b[idxD + Ns] = cadd(a1, a3);
-                b[idxD + 2 * Ns] = csub(a0, a2);
-                b[idxD + 3 * Ns] = csub(a1, a3);
+                a[idxD] = cadd(a0, a2);
+                a[idxD + Ns] = cadd(a1, a3);
+                a[idxD + 2 * Ns] = csub(a0, a2);
+                a[idxD + 3 * Ns] = csub(a1, a3);
             } else {
-                double2 v0 = a[j];
-                double2 v1 = a[j + nbf];
+                double2 v0 = v[b][0], v1 = v[b][1];
                 if (tstep != 0) {
-                    double2 w1 = tw[tstep];
+                    double2 w1 = twiddle(tw, tstep, half);
                     if (INV) {
                         w1 = cconj(w1);
                     }
                     v1 = cmul(v1, w1);
                 }
-                b[idxD] = cadd(v0, v1);
-                b[idxD + Ns] = csub(v0, v1);
+                a[idxD] = cadd(v0, v1);
+                a[idxD + Ns] = csub(v0, v1);
             }
         }
         __syncthreads();
-        double2* t = a;
-        a = b;
-        b = t;
         Ns *= R;
     }
-    return a;
 }
 
 // Spectrum F_k (k = 0..M) of the real input from Z = FFT_M(z):

@@ -15,28 +15,28 @@
 namespace vfps_impl {
 
 template <bool WAKE, bool CSR>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 field_kernel(FieldParams p)
 {
     extern __shared__ double2 lds[];
     field_body<WAKE, CSR>(p, blockIdx.x, lds);
 }
 
-size_t field_lds_bytes(uint32_t M) { return sizeof(double2) * 2 * (size_t)M; }
+size_t field_lds_bytes(uint32_t M) { return sizeof(double2) * (M + M / 2); }
 
 int launch_field(const FieldParams& p, bool wake, bool csr, uint32_t blocks,
                  hipStream_t st)
 {
     const size_t lds = field_lds_bytes(p.M);
     if (wake && csr) {
-        hipLaunchKernelGGL((field_kernel<true, true>), dim3(1), dim3(512), lds,
+        hipLaunchKernelGGL((field_kernel<true, true>), dim3(1), dim3(FFT_THREADS), lds,
                            st, p);
     } else if (wake) {
-        hipLaunchKernelGGL((field_kernel<true, false>), dim3(1), dim3(512),
+        hipLaunchKernelGGL((field_kernel<true, false>), dim3(1), dim3(FFT_THREADS),
                            lds, st, p);
     } else {
         hipLaunchKernelGGL((field_kernel<false, true>), dim3(blocks),
-                           dim3(512), lds, st, p);
+                           dim3(FFT_THREADS), lds, st, p);
     }
     VFPS_HIP(hipGetLastError());
     return VFPS_OK;
@@ -149,9 +149,9 @@ int vfps_field_create(vfps_mesh* ps, vfps_impedance* z,
     VFPS_CHECK_ARG(nbuckets == ps->nb,
                    "ElectricField: one bucket number per bunch expected");
     const uint32_t nmax = z->n;
-    VFPS_CHECK_ARG(nmax >= 4 && (nmax & (nmax - 1)) == 0,
+    VFPS_CHECK_ARG(nmax >= 4 && nmax <= 8192 && (nmax & (nmax - 1)) == 0,
                    "ElectricField: FFT length (impedance size) must be a "
-                   "power of two >= 4");
+                   "power of two in [4, 8192]");
     for (uint32_t b = 0; b < nbuckets; b++) {
         VFPS_CHECK_ARG((size_t)buckets[b] * spacing_bins + ps->n <= nmax,
                        "ElectricField: bunch does not fit the padded array");

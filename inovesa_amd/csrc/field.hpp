@@ -28,6 +28,7 @@ struct FieldParams {
     uint32_t* counter;        // optional step counter (history slot)
     float* csr_hist;          // optional per-step CSR power history
     uint32_t hist_cap;        // history capacity (steps)
+    unsigned long long* tstamp;  // optional phase timestamps (block 0)
 };
 
 FieldParams field_params(vfps_field* f);

@@ -16,18 +16,58 @@ namespace vfps_impl {
 // The transforms run in FP64 (fft.hpp); the spectrum F, the product
 // W = Z F and the wake potential are rounded to FP32 where the reference
 // holds them as float, and the float arithmetic on them follows the
-// reference.  lds: 2*M double2.
+// reference.  lds: M + M/2 double2 (data, twiddle half table).  Runs with
+// FFT_THREADS threads.
+constexpr int FIELD_PAIRS = 3;  // (k, M-k) pairs per thread: M <= 4096
+
+#define VFPS_FSTAMP(k)                                                        \
+    do {                                                                      \
+        if (p.tstamp && blockIdx.x == 0) {                                    \
+            __syncthreads();                                                  \
+            if (threadIdx.x == 0) p.tstamp[k] = wall_clock64();               \
+        }                                                                     \
+    } while (0)
+
 template <bool WAKE, bool CSR>
 __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
                            double2* lds)
 {
     __shared__ double red[16];
+    VFPS_FSTAMP(0);
     const uint32_t M = p.M;
     double2* A = lds;
-    double2* B = lds + M;
+    double2* twl = lds + M;
     double* Ad = reinterpret_cast<double*>(A);
     for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
         A[i] = make_double2(0.0, 0.0);
+    }
+    if (WAKE || CSR) {
+        for (uint32_t i = threadIdx.x; i < M / 2; i += blockDim.x) {
+            twl[i] = p.tw_m[i];
+        }
+    }
+    // this thread's (k, M-k) pairs: tables fetched before the transform so
+    // their latency hides behind it
+    float2 zk[FIELD_PAIRS], zkp[FIELD_PAIRS];
+    float ck[FIELD_PAIRS], ckp[FIELD_PAIRS], rk[FIELD_PAIRS], rkp[FIELD_PAIRS];
+    double2 wn[FIELD_PAIRS];
+#pragma unroll
+    for (int j = 0; j < FIELD_PAIRS; j++) {
+        const uint32_t k = threadIdx.x + j * blockDim.x;
+        const uint32_t kp = M - k;
+        if ((WAKE || CSR) && k <= M / 2) {
+            wn[j] = p.tw_n[k];
+            if (WAKE) {
+                zk[j] = p.Z[k];
+                zkp[j] = p.Z[kp];
+            }
+            if (CSR) {
+                ck[j] = p.Zc[k].x;
+                ckp[j] = p.Zc[kp].x;
+                rk[j] = p.renorm[k];
+                rkp[j] = p.renorm[kp];
+            }
+        }
     }
     __syncthreads();
     // padBunchProfiles (src/PS/ElectricField.cpp:255-263)
@@ -76,57 +116,64 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
         }
         return;
     }
-    double2* Zf = fft_lds<false>(A, B, M, p.tw_m);
-    double2* other = (Zf == A) ? B : A;
+    VFPS_FSTAMP(1);
+    fft_lds<false>(A, M, twl);
+    VFPS_FSTAMP(2);
+    double2* Zf = A;
     // CSR power: each bin's float term as the reference forms it, summed
     // in FP64 (the reference's sequential float sum carries a rounding
     // error of up to nmax*eps; the FP64 sum is the exact sum of the terms)
     double csr = 0.0;
     const size_t sbase = CSR ? (size_t)(WAKE ? 0 : block_bunch) * 2 * M : 0;
     // CSR spectrum term of one bin from the float spectrum value
-    auto csr_bin = [&](uint32_t k, float2 f) {
+    auto csr_bin = [&](uint32_t k, float2 f, float rn, float zc) {
         const float a = hypotf(f.x, f.y);  // std::norm via std::abs
-        const float s = p.renorm[k] * p.Zc[k].x * (a * a);
+        const float s = rn * zc * (a * a);
         if (p.spectrum) {
             p.spectrum[sbase + k] = s;
         }
         return (double)(p.fdelta * s);
     };
-    for (uint32_t k = threadIdx.x; k <= M / 2; k += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < FIELD_PAIRS; j++) {
+        const uint32_t k = threadIdx.x + j * blockDim.x;
+        if (k > M / 2) {
+            continue;
+        }
         if (k == 0) {
             const double2 z0 = Zf[0];
             const float f0 = (float)(z0.x + z0.y);  // F_0 (real)
             const float fm = (float)(z0.x - z0.y);  // F_M (real)
             if (CSR) {
-                csr += csr_bin(0, make_float2(f0, 0.0f));
-                csr += csr_bin(M, make_float2(fm, 0.0f));
+                csr += csr_bin(0, make_float2(f0, 0.0f), rk[j], ck[j]);
+                csr += csr_bin(M, make_float2(fm, 0.0f), rkp[j], ckp[j]);
             }
             if (WAKE) {
                 // W_0 = Z_0 F_0; c2r ignores its imaginary part.  W_M is
                 // never assigned by the reference and is taken as zero.
-                const float2 w0 = cmulf(p.Z[0], make_float2(f0, 0.0f));
+                const float2 w0 = cmulf(zk[j], make_float2(f0, 0.0f));
                 Zf[0] = make_double2((double)w0.x, (double)w0.x);
             }
         } else {
             const uint32_t kp = M - k;
             double2 fk, fkp;
-            r2c_pair(Zf[k], Zf[kp], p.tw_n[k], &fk, &fkp);
+            r2c_pair(Zf[k], Zf[kp], wn[j], &fk, &fkp);
             const float2 f1 = make_float2((float)fk.x, (float)fk.y);
             const float2 f2 = make_float2((float)fkp.x, (float)fkp.y);
             if (CSR) {
-                csr += csr_bin(k, f1);
+                csr += csr_bin(k, f1, rk[j], ck[j]);
                 if (kp != k) {
-                    csr += csr_bin(kp, f2);
+                    csr += csr_bin(kp, f2, rkp[j], ckp[j]);
                 }
             }
             if (WAKE) {
-                const float2 w1 = cmulf(p.Z[k], f1);
-                const float2 w2 = cmulf(p.Z[kp], f2);
+                const float2 w1 = cmulf(zk[j], f1);
+                const float2 w2 = cmulf(zkp[j], f2);
                 const double2 wk = make_double2(w1.x, w1.y);
                 const double2 wkp = make_double2(w2.x, w2.y);
                 if (kp != k) {
                     double2 yk, ykp;
-                    c2r_pair(wk, wkp, p.tw_n[k], &yk, &ykp);
+                    c2r_pair(wk, wkp, wn[j], &yk, &ykp);
                     Zf[k] = yk;
                     Zf[kp] = ykp;
                 } else {
@@ -158,8 +205,10 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
         return;
     }
     __syncthreads();
-    double2* y = fft_lds<true>(Zf, other, M, p.tw_m);
-    const double* yd = reinterpret_cast<const double*>(y);
+    VFPS_FSTAMP(3);
+    fft_lds<true>(A, M, twl);
+    VFPS_FSTAMP(4);
+    const double* yd = reinterpret_cast<const double*>(A);
     for (uint32_t b = 0; b < p.nb; b++) {
         const uint32_t base = p.buckets[b] * p.spacing;
         for (uint32_t x = threadIdx.x; x < p.n; x += blockDim.x) {
@@ -170,6 +219,7 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
             }
         }
     }
+    VFPS_FSTAMP(5);
 }
 
 }  // namespace vfps_impl

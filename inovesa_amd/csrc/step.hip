@@ -17,6 +17,7 @@
 // even number of steps (the fused path ping-pongs two buffers).
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <memory>
 
 #include "common.hpp"
@@ -50,6 +51,19 @@ __device__ __forceinline__ void row_kick(const float* __restrict__ src,
     }
 }
 
+// most mesh columns one thread handles (n <= 1024 with >= 256 threads)
+constexpr int MAPS_MAX_COLS = 4;
+// float4 loads one thread keeps in flight while filling the tile
+constexpr int MAPS_LOAD_BATCH = 8;
+
+#define VFPS_STAMP(k)                                                         \
+    do {                                                                      \
+        if (d.tstamp && blockIdx.x == 0) {                                    \
+            __syncthreads();                                                  \
+            if (threadIdx.x == 0) d.tstamp[k] = wall_clock64();               \
+        }                                                                     \
+    } while (0)
+
 template <int IT, int IPFP, bool WAKE, int SCHEME, bool CLAMP>
 __global__ void __launch_bounds__(512)
 maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
@@ -71,27 +85,76 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
     if (x0 >= (int)n) {
         return;
     }
+    VFPS_STAMP(0);
     const int rows_out = min((int)T, (int)n - x0);
     const int hlo = (int)d.hlo;
     const int rbase = x0 - hlo;
     const int R = rows_out + hlo + (int)d.hhi;
     const int stride = (int)(T + 2 * hmax);  // rows per LDS buffer
+    const uint32_t tstride = n + 4;           // padded row of the output tile
     float* bufA = lds;
-    float* bufB = lds + (size_t)stride * n;
+    float* bufB = bufA + (size_t)stride * n;
+    float* tile_out = bufB + (size_t)stride * n;  // [T][n+4] products
+    float* offw = tile_out + (size_t)T * tstride;  // [stride] wake offsets
+    float* offr = offw + stride;                   // [stride] RF offsets
     const float* in = parity ? d.mesh1 : d.mesh0;
     float* out = parity ? d.mesh0 : d.mesh1;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6, nw = bs >> 6;
+    const int32_t half = (int32_t)(n / 2);
 
-    // phase 1: rows [rbase, rbase+R) of the input into LDS
+    // phase 1: every global read the block needs, issued together
+    //  - per-row kick offsets of the tile's rows
+    if ((int)tid < R) {
+        const int r = rbase + (int)tid;
+        if (r >= 0 && r < (int)n) {
+            if (WAKE) offw[tid] = d.wake_off[r];
+            if (SCHEME == 0) offr[tid] = d.rf_off[r];
+        }
+    }
+    //  - per-column tables (drift offset, Fokker-Planck stencil, Simpson
+    //    weight) of this thread's columns, kept in registers
+    const int ncols = (int)((n + bs - 1) / bs);
+    float col_drift[MAPS_MAX_COLS], col_ws[MAPS_MAX_COLS];
+    uint32_t col_fi[MAPS_MAX_COLS][IPFP > 0 ? IPFP : 1];
+    float col_fw[MAPS_MAX_COLS][IPFP > 0 ? IPFP : 1];
+#pragma unroll
+    for (int c = 0; c < MAPS_MAX_COLS; c++) {
+        const uint32_t y = tid + (uint32_t)c * bs;
+        if (c < ncols && y < n) {
+            if (SCHEME == 0) col_drift[c] = d.drift_off[y];
+            col_ws[c] = d.ws[y];
+#pragma unroll
+            for (int k = 0; k < IPFP; k++) {
+                col_fi[c][k] = d.fp_i[(size_t)k * n + y];
+                col_fw[c][k] = d.fp_w[(size_t)k * n + y];
+            }
+        }
+    }
+    //  - rows [rbase, rbase+R) of the input, MAPS_LOAD_BATCH float4 loads
+    //    in flight per thread
     if ((n & 3u) == 0u) {
         const uint32_t n4 = n >> 2;
         const float4* in4 = reinterpret_cast<const float4*>(in);
         float4* a4 = reinterpret_cast<float4*>(bufA);
-        for (uint32_t i = tid; i < (uint32_t)R * n4; i += bs) {
-            const int r = rbase + (int)(i / n4);
-            if (r >= 0 && r < (int)n) {
-                a4[i] = in4[(size_t)r * n4 + (i % n4)];
+        const uint32_t total = (uint32_t)R * n4;
+        for (uint32_t base = 0; base < total; base += bs * MAPS_LOAD_BATCH) {
+            float4 v[MAPS_LOAD_BATCH];
+#pragma unroll
+            for (int b = 0; b < MAPS_LOAD_BATCH; b++) {
+                const uint32_t i = base + tid + (uint32_t)b * bs;
+                const int r = rbase + (int)(i / n4);
+                if (i < total && r >= 0 && r < (int)n) {
+                    v[b] = in4[(size_t)r * n4 + (i % n4)];
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < MAPS_LOAD_BATCH; b++) {
+                const uint32_t i = base + tid + (uint32_t)b * bs;
+                const int r = rbase + (int)(i / n4);
+                if (i < total && r >= 0 && r < (int)n) {
+                    a4[i] = v[b];
+                }
             }
         }
     } else {
@@ -103,6 +166,7 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
         }
     }
     __syncthreads();
+    VFPS_STAMP(1);
     float* cur = bufA;
     float* oth = bufB;
     // phase 2: wake kick along y (WakePotentialMap::apply), row by row
@@ -113,13 +177,14 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
                 continue;
             }
             row_kick<IT>(cur + (size_t)row * n, oth + (size_t)row * n, n,
-                         d.wake_off[r], lane);
+                         offw[row], lane);
         }
         __syncthreads();
         float* t = cur;
         cur = oth;
         oth = t;
     }
+    VFPS_STAMP(2);
     if constexpr (SCHEME == 0) {
         // phase 3: RF kick along y (RFKickMap::apply)
         for (int row = (int)wave; row < R; row += (int)nw) {
@@ -128,18 +193,23 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
                 continue;
             }
             row_kick<IT>(cur + (size_t)row * n, oth + (size_t)row * n, n,
-                         d.rf_off[r], lane);
+                         offr[row], lane);
         }
         __syncthreads();
         float* t = cur;
         cur = oth;
         oth = t;
+        VFPS_STAMP(3);
         // phase 4: drift along x (DriftMap::apply), one column per thread
-        const int32_t half = (int32_t)(n / 2);
-        for (uint32_t y = tid; y < n; y += bs) {
+#pragma unroll
+        for (int c = 0; c < MAPS_MAX_COLS; c++) {
+            const uint32_t y = tid + (uint32_t)c * bs;
+            if (c >= ncols || y >= n) {
+                continue;
+            }
             uint32_t idx[IT];
             float w[IT];
-            kick_taps<IT>(d.drift_off[y], n, idx, w);
+            kick_taps<IT>(col_drift[c], n, idx, w);
             for (int orow = 0; orow < rows_out; orow++) {
                 const uint32_t x = (uint32_t)(x0 + orow);
                 float v = 0.0f;
@@ -168,7 +238,6 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
             uint32_t idx[IT];
             float w[IT];
             kick_taps<IT>(d.wake_off[0], n, idx, w);
-            const int32_t half = (int32_t)(n / 2);
             float v = 0.0f;
 #pragma unroll
             for (int j = 0; j < IT; j++) {
@@ -181,6 +250,7 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
         } else {
             s00 = in[0];
         }
+        VFPS_STAMP(3);
         const RotGeom g = d.rot;
         const float* src = cur;
         for (uint32_t i = tid; i < (uint32_t)rows_out * n; i += bs) {
@@ -202,25 +272,18 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
         }
     }
     __syncthreads();
+    VFPS_STAMP(4);
     // phase 5: Fokker-Planck along y (FokkerPlanckMap::apply), the output
-    // store and the x projection (PhaseSpace::updateXProjection)
-    uint32_t slot = 0xFFFFFFFFu;
-    if (d.prof_hist) {
-        const uint32_t c = *d.counter;
-        if (c >= 1 && c - 1 < d.hist_cap) {
-            slot = c - 1;
+    // store and the products mesh*weight of the x projection, one column
+    // per thread with the column's stencil in registers
+#pragma unroll
+    for (int c = 0; c < MAPS_MAX_COLS; c++) {
+        const uint32_t y = tid + (uint32_t)c * bs;
+        if (c >= ncols || y >= n) {
+            continue;
         }
-    }
-    // rows of the output tile, padded to n+1 floats so that one thread per
-    // row reads without bank conflicts
-    float* tile_out = lds + (size_t)2 * stride * n;
-    const bool seq = d.seq_proj != 0;
-    for (int orow = (int)wave; orow < rows_out; orow += (int)nw) {
-        const uint32_t x = (uint32_t)(x0 + orow);
-        const float* src = oth + (size_t)orow * n;
-        float* dst = out + (size_t)x * n;
-        float acc = 0.0f;
-        for (uint32_t y = lane; y < n; y += 64) {
+        for (int orow = 0; orow < rows_out; orow++) {
+            const float* src = oth + (size_t)orow * n;
             float v;
             if constexpr (IPFP == 0) {
                 v = src[y];
@@ -228,35 +291,49 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
                 v = 0.0f;
 #pragma unroll
                 for (int k = 0; k < IPFP; k++) {
-                    v = v + src[d.fp_i[(size_t)k * n + y]] * d.fp_w[(size_t)k * n + y];
+                    v = v + src[col_fi[c][k]] * col_fw[c][k];
                 }
             }
-            dst[y] = v;
-            if (seq) {
-                tile_out[(size_t)orow * (n + 1) + y] = v;
-            } else {
-                acc = acc + v * d.ws[y];
-            }
-        }
-        if (!seq) {
-            acc = wave_sum(acc);
-            if (lane == 0) {
-                d.proj[x] = acc;
-                if (slot != 0xFFFFFFFFu) {
-                    d.prof_hist[(size_t)slot * n + x] = acc;
-                }
-            }
+            out[(size_t)(x0 + orow) * n + y] = v;
+            tile_out[(size_t)orow * tstride + y] = v * col_ws[c];
         }
     }
-    if (seq) {
+    uint32_t slot = 0xFFFFFFFFu;
+    if (d.prof_hist) {
+        const uint32_t cnt = *d.counter;
+        if (cnt >= 1 && cnt - 1 < d.hist_cap) {
+            slot = cnt - 1;
+        }
+    }
+    __syncthreads();
+    VFPS_STAMP(5);
+    // phase 6: the x projection (PhaseSpace::updateXProjection)
+    if (d.seq_proj) {
         // the reference's std::inner_product order: y = 0..n-1, one thread
-        // per row
-        __syncthreads();
+        // per row, loads batched ahead of the dependent adds
         if ((int)tid < rows_out) {
-            const float* r = tile_out + (size_t)tid * (n + 1);
+            const float* r = tile_out + (size_t)tid * tstride;
             float acc = 0.0f;
-            for (uint32_t y = 0; y < n; y++) {
-                acc = acc + r[y] * d.ws[y];
+            uint32_t y = 0;
+            if ((n & 3u) == 0u) {
+                const float4* r4 = reinterpret_cast<const float4*>(r);
+                for (; y + 16 <= n; y += 16) {
+                    float4 q[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        q[k] = r4[(y >> 2) + k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        acc = acc + q[k].x;
+                        acc = acc + q[k].y;
+                        acc = acc + q[k].z;
+                        acc = acc + q[k].w;
+                    }
+                }
+            }
+            for (; y < n; y++) {
+                acc = acc + r[y];
             }
             const uint32_t x = (uint32_t)x0 + tid;
             d.proj[x] = acc;
@@ -264,7 +341,24 @@ maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
                 d.prof_hist[(size_t)slot * n + x] = acc;
             }
         }
+    } else {
+        for (int orow = (int)wave; orow < rows_out; orow += (int)nw) {
+            const float* r = tile_out + (size_t)orow * tstride;
+            float acc = 0.0f;
+            for (uint32_t y = lane; y < n; y += 64) {
+                acc = acc + r[y];
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) {
+                const uint32_t x = (uint32_t)(x0 + orow);
+                d.proj[x] = acc;
+                if (slot != 0xFFFFFFFFu) {
+                    d.prof_hist[(size_t)slot * n + x] = acc;
+                }
+            }
+        }
     }
+    VFPS_STAMP(6);
 }
 
 template <int IT, int IPFP, bool WAKE, int SCHEME, bool CLAMP>
@@ -308,7 +402,8 @@ static const void* maps_kernel_for(const MapsVariant& v)
 
 size_t fused_lds_bytes(uint32_t n, uint32_t T, uint32_t hmax)
 {
-    return sizeof(float) * (2 * (size_t)(T + 2 * hmax) * n + (size_t)T * (n + 1));
+    const size_t stride = T + 2 * hmax;
+    return sizeof(float) * (2 * stride * n + (size_t)T * (n + 4) + 2 * stride);
 }
 
 int fused_set_lds_limit(const MapsVariant& v, size_t bytes)
@@ -346,7 +441,7 @@ int launch_fused_maps(const MapsVariant& v, const InstDesc* descs_d,
 // ---- fields kernel (one workgroup per instance) ------------------------------
 
 template <bool WAKE, bool CSR>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 fields_kernel(const FieldParams* __restrict__ params)
 {
     extern __shared__ double2 flds[];
@@ -374,7 +469,7 @@ int launch_fields(bool wake, bool csr, const FieldParams* params_d,
                                      (int)lds));
     }
     void* args[] = {(void*)&params_d};
-    VFPS_HIP(hipLaunchKernel(fn, dim3(ninst), dim3(512), args, lds, st));
+    VFPS_HIP(hipLaunchKernel(fn, dim3(ninst), dim3(FFT_THREADS), args, lds, st));
     return VFPS_OK;
 }
 
@@ -689,7 +784,7 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
         if (rc != VFPS_OK) return rc;
         s->M = s->wfield->nmax / 2;
     } else if (desc->use_csr) {
-        VFPS_CHECK_ARG(zcsr->n >= n && zcsr->n >= 4 &&
+        VFPS_CHECK_ARG(zcsr->n >= n && zcsr->n >= 4 && zcsr->n <= 8192 &&
                            (zcsr->n & (zcsr->n - 1)) == 0,
                        "vfps_step_create: CSR impedance length must be a "
                        "power of two >= the mesh size");
@@ -905,6 +1000,52 @@ int vfps_step_set_mesh(vfps_step* s, const float* mesh)
     int rc = launch_project_x(s->t1, s->t1->data, s->t1->proj);
     if (rc != VFPS_OK) return rc;
     VFPS_HIP(hipStreamSynchronize(s->dev->stream));
+    return VFPS_OK;
+}
+
+int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
+{
+    VFPS_CHECK_ARG(s && ns_out && nout >= 16, "vfps_step_phases: bad argument");
+    VFPS_CHECK_ARG(s->fused, "vfps_step_phases: needs the fused step");
+    hipStream_t st = s->dev->stream;
+    VFPS_HIP(hipStreamSynchronize(st));
+    unsigned long long* buf = nullptr;
+    VFPS_HIP(hipMalloc(&buf, sizeof(unsigned long long) * 16));
+    VFPS_HIP(hipMemset(buf, 0, sizeof(unsigned long long) * 16));
+    // point both descriptors at the stamp buffer for one step
+    unsigned long long* mstamp = buf;
+    unsigned long long* fstamp = buf + 8;
+    VFPS_HIP(hipMemcpy(reinterpret_cast<char*>(s->inst_d) + offsetof(InstDesc, tstamp),
+                       &mstamp, sizeof(mstamp), hipMemcpyHostToDevice));
+    VFPS_HIP(hipMemcpy(reinterpret_cast<char*>(s->fparams_d) + offsetof(FieldParams, tstamp),
+                       &fstamp, sizeof(fstamp), hipMemcpyHostToDevice));
+    int rc = step_launch_one(s, s->parity);
+    s->parity ^= 1u;
+    hipError_t e = hipStreamSynchronize(st);
+    unsigned long long h[16] = {0};
+    hipMemcpy(h, buf, sizeof(h), hipMemcpyDeviceToHost);
+    unsigned long long* nul = nullptr;
+    hipMemcpy(reinterpret_cast<char*>(s->inst_d) + offsetof(InstDesc, tstamp),
+              &nul, sizeof(nul), hipMemcpyHostToDevice);
+    hipMemcpy(reinterpret_cast<char*>(s->fparams_d) + offsetof(FieldParams, tstamp),
+              &nul, sizeof(nul), hipMemcpyHostToDevice);
+    hipFree(buf);
+    if (rc != VFPS_OK) return rc;
+    if (e != hipSuccess) return fail(VFPS_ERR_HIP, hipGetErrorString(e));
+    int khz = 0;
+    VFPS_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
+                                   s->dev->ordinal));
+    const double ns_per_tick = khz > 0 ? 1e6 / khz : 10.0;
+    // [0..6]: maps kernel phase ends relative to its start;
+    // [8..13]: field kernel phase ends relative to its start; 15: tick ns
+    for (uint32_t i = 0; i < 16; i++) ns_out[i] = 0;
+    for (int i = 1; i < 7; i++) {
+        if (h[i] && h[0]) ns_out[i] = (double)(h[i] - h[0]) * ns_per_tick;
+    }
+    for (int i = 9; i < 14; i++) {
+        if (h[i] && h[8]) ns_out[i] = (double)(h[i] - h[8]) * ns_per_tick;
+    }
+    ns_out[15] = ns_per_tick;
     return VFPS_OK;
 }
 

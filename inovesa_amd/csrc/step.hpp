@@ -28,6 +28,7 @@ struct InstDesc {
     const uint32_t* counter;      // steps counted by the field kernel
     uint32_t* err;                // set if a tap falls outside the halo
     uint32_t seq_proj;            // 1: x projection in the reference's order
+    unsigned long long* tstamp;   // optional phase timestamps of workgroup 0
 };
 
 // kernel variant key: scheme (0 kick-drift, 1 rotation), interpolation,

@@ -565,6 +565,12 @@ class Step:
              C.byref(c))
         return a.value, b.value, c.value
 
+    def phases(self):
+        """Per-phase end times (ns) of one fused step: (maps, field)."""
+        out = np.zeros(16, np.float64)
+        call("vfps_step_phases", self.h, _p(out), 16)
+        return out[1:7].copy(), out[9:14].copy()
+
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
