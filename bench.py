@@ -9,8 +9,9 @@ device every step.  One "step" is one time step of the mesh.
 
 Prints one JSON line.  value = mesh-cell updates per second
 (n^2 * steps / wall time of the timed steps, state resident on the GPU).
-roofline: the fused map kernel (8 bytes of mesh traffic per cell) timed
-with HIP events on its stream.  cpu_baseline: the oracle (the FP32 C
+roofline: the step kernel (the whole step in one launch; 8 bytes of mesh
+traffic per cell) timed with HIP events on its stream.  fft_wake: the same
+workload with the wake and CSR power from the FFT field kernel.  cpu_baseline: the oracle (the FP32 C
 restatement of the reference CPU path) on one host core, on a bounded
 sample of the same workload.
 """
@@ -93,15 +94,32 @@ def main():
     assert len(cnt_csr) == args.warmup + args.steps
     value = n * n * args.steps / dt
 
-    ms_wake, ms_maps, _ = st.timing(200)
+    cfg = st.config()
+    _, ms_step_kernel, _ = st.timing(400)
     bytes_per_launch = 8.0 * n * n
-    achieved = bytes_per_launch / (ms_maps * 1e-3) / 1e9
+    achieved = bytes_per_launch / (ms_step_kernel * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": load_traffic(f"maps_kd_{n}"),
-                "kernel": "maps_kernel (wake+RF kick, drift, Fokker-Planck, "
-                          "projection)",
-                "kernel_ms": ms_maps, "field_kernel_ms": ms_wake}
+                "traffic": load_traffic(f"step_kd_{n}"),
+                "kernel": "step_kernel (direct wake + CSR power, wake kick, "
+                          "RF kick, drift, Fokker-Planck, projection: the "
+                          "whole step)",
+                "kernel_ms": ms_step_kernel,
+                "bytes_per_launch": bytes_per_launch}
+
+    # the same workload with the FFT wake (field kernel + step kernel)
+    fft = Step(dict(desc, wake_method=1), init, zw, zc)
+    fft.run(640)
+    fft.dev.sync()
+    kf = min(args.steps, 6400)
+    t0 = time.perf_counter()
+    fft.run(kf)
+    fft.dev.sync()
+    dtf = time.perf_counter() - t0
+    ms_field, ms_maps_fft, _ = fft.timing(200)
+    fft_line = {"value": n * n * kf / dtf, "steps": kf,
+                "ms_per_step": dtf * 1e3 / kf, "field_kernel_ms": ms_field,
+                "step_kernel_ms": ms_maps_fft}
 
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -114,8 +132,10 @@ def main():
                                   "2000 steps/Ts, profile + CSR power "
                                   "recorded every step",
                       "n": n, "nmax": zw.n, "steps_per_ts": 2000,
-                      "Ib_A": d["Ib"], "tile_rows": int(desc.get("tile_rows", 0))},
-           "roofline": roofline}
+                      "Ib_A": d["Ib"], "wake": "direct (FP64 Toeplitz)",
+                      "tile_rows": cfg["tile_rows"],
+                      "threads": cfg["threads"]},
+           "roofline": roofline, "fft_wake": fft_line}
     if not args.no_cpu:
         cb = cpu_baseline(desc, zw, zc, init, args.cpu_budget)
         cb["ratio_gpu_over_cpu"] = value / cb["value"]

@@ -170,7 +170,7 @@ VFPS_API int vfps_map_update(vfps_map* map);
 VFPS_API int vfps_map_apply(vfps_map* map);
 VFPS_API int vfps_map_destroy(vfps_map* map);
 
-/* ---- Impedance (inc/Z/Impedance.hpp; src/Z/*) --------------------------- */
+/* ---- Impedance (inc/Z/Impedance.hpp; src/Z/) --------------------------- */
 /* Impedance(nfreqs, f_max): a zero table (src/Z/Impedance.cpp:48-57) */
 VFPS_API int vfps_impedance_create(vfps_dev* dev, uint32_t nfreqs,
                                    float f_max, vfps_impedance** out);
@@ -316,7 +316,26 @@ typedef struct {
     uint32_t tile_rows;       /* 0: autotuned; rows per workgroup of the fused kernel */
     int32_t fast_projection;  /* 0: x projection summed in the reference's
                                  order (bit-exact); 1: wavefront tree sum */
+    uint32_t wake_method;     /* fused step: 0 default (2), 1 FFT field kernel
+                                 then the maps kernel, 2 direct: one kernel
+                                 per step, wake and CSR power as FP64
+                                 Toeplitz products of the profile */
+    uint32_t threads;         /* fused step workgroup size (0: default) */
 } vfps_step_desc;
+
+/* The FP64 Toeplitz kernels of the direct wake and CSR forms (host only,
+ * no device needed).  With N = nmax, M = N/2, theta = 2 pi k d / N:
+ *   g[d] = Re z_0 + 2 sum_{k=1}^{M-1} (Re z_k cos theta - Im z_k sin theta),
+ * the C2R of z with its Nyquist bin zero, so that the wake potential of
+ * src/PS/ElectricField.cpp:215-241 before scaling is sum_x' g[x-x'] p[x'];
+ *   h[d] = sum_{k=0}^{M} fdelta renorm_k Re zc_k cos theta,
+ * so that the CSR power of src/PS/ElectricField.cpp:133-184 is
+ * sum_{x,x'} p[x] h[x-x'] p[x'].  g, h: 2n-1 doubles, index d + n - 1;
+ * either may be NULL. */
+VFPS_API int vfps_toeplitz_kernels(uint32_t nmax, uint32_t n,
+                                   const vfps_cplx* z, const vfps_cplx* zc,
+                                   const float* renorm, float fdelta,
+                                   double* g, double* h);
 
 VFPS_API int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
                               const float* init, vfps_impedance* zwake,
@@ -346,6 +365,48 @@ VFPS_API int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
  * field kernel (loads, forward FFT, spectrum pass, inverse FFT, output),
  * nanoseconds since the kernel's first stamp; [15] the clock period */
 VFPS_API int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout);
+/* the configuration in use: out[0] fused, [1] wake method (1 FFT, 2 direct),
+ * [2] tile rows, [3] threads, [4] LDS buffer rows, [5] LDS bytes (nout >= 6) */
+VFPS_API int vfps_step_config(vfps_step* s, uint32_t* out, uint32_t nout);
+
+/* ---- batch of independent meshes ------------------------------------- *
+ * No reference counterpart: a scan over currents or chamber gaps runs the
+ * time loop of src/main.cpp:948-1053 once per point.  Here ninst instances
+ * of any mesh sizes (each described as for vfps_step_create, with its own
+ * initial mesh and impedances) advance together: one step kernel launch per
+ * step covers every row tile of every instance (one launch per kernel
+ * variant and workgroup size), captured into a hipGraph.  Each instance
+ * equals a fused single-mesh step with the direct wake bit for bit. */
+VFPS_API int vfps_batch_create(vfps_dev* dev, uint32_t ninst,
+                               const vfps_step_desc* descs,
+                               const float* const* inits,
+                               vfps_impedance* const* zwake,
+                               vfps_impedance* const* zcsr, vfps_batch** out);
+VFPS_API int vfps_batch_destroy(vfps_batch* b);
+VFPS_API int vfps_batch_run(vfps_batch* b, uint32_t nsteps);
+VFPS_API int vfps_batch_record(vfps_batch* b, uint32_t capacity,
+                               int want_profiles);
+VFPS_API int vfps_batch_get_history(vfps_batch* b, uint32_t inst,
+                                    uint32_t* count, float* csr_power,
+                                    float* profiles);
+VFPS_API int vfps_batch_get_mesh(vfps_batch* b, uint32_t inst, float* mesh,
+                                 float* proj, float* wake);
+/* instances, launch groups and mesh cells per step */
+VFPS_API int vfps_batch_info(vfps_batch* b, uint32_t* ninst,
+                             uint32_t* ngroups, uint64_t* cells);
+/* vfps_step_config of one instance */
+VFPS_API int vfps_batch_config(vfps_batch* b, uint32_t inst, uint32_t* out,
+                               uint32_t nout);
+/* average time of one batch step (all launches), HIP events on the device
+ * stream; advances the state by nsteps */
+VFPS_API int vfps_batch_timing(vfps_batch* b, uint32_t nsteps, float* ms_step);
+/* Autotuner: per mesh size, time every tile height T in {1,2,4,8} and
+ * workgroup size in {128,256,512} that fits (steps steps each, on that
+ * size's instances alone) and keep the fastest.  The instances' state is
+ * restored afterwards; results do not depend on the choice.  table (cap
+ * rows of 5 floats: n, T, threads, us/step, chosen) may be NULL. */
+VFPS_API int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
+                                 uint32_t cap, uint32_t* nrows);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,6 @@
 """ctypes binding of libvfps.so (the C ABI in include/vfps_gpu.h).
 
-The library is built in-tree by inovesa_amd.build (hipcc, gfx950).  There is
+The library is built in-tree by inovesa_amd/build.py (hipcc, gfx950).  There is
 no CPU fallback: if the shared object is missing, importing this module
 raises, and every call that fails on the device raises VfpsError with the
 library's message.
@@ -26,7 +26,7 @@ class VfpsInvalid(VfpsError, ValueError):
 
 if not os.path.exists(LIBPATH):
     raise ImportError(
-        f"{LIBPATH} is missing: build it with `python -m inovesa_amd.build` "
+        f"{LIBPATH} is missing: build it with `python inovesa_amd/build.py` "
         "(hipcc, gfx950).  There is no CPU fallback.")
 
 lib = C.CDLL(LIBPATH)
@@ -77,7 +77,8 @@ class StepDesc(C.Structure):
                 ("e1", _f32), ("deriv", _u32), ("use_wake", _i32),
                 ("spacing_bins", _u32), ("wakescaling", _f32),
                 ("use_csr", _i32), ("fc", _f32), ("fused", _i32),
-                ("tile_rows", _u32), ("fast_projection", _i32)]
+                ("tile_rows", _u32), ("fast_projection", _i32),
+                ("wake_method", _u32), ("threads", _u32)]
 
 
 _SIGS = {
@@ -155,6 +156,20 @@ _SIGS = {
     "vfps_step_set_mesh": (_int, [_vp, _vp]),
     "vfps_step_timing": (_int, [_vp, _u32, _vp, _vp, _vp]),
     "vfps_step_phases": (_int, [_vp, _vp, _u32]),
+    "vfps_step_config": (_int, [_vp, _vp, _u32]),
+    "vfps_batch_create": (_int, [_vp, _u32, C.POINTER(StepDesc), _vp, _vp,
+                                 _vp, _vp]),
+    "vfps_batch_destroy": (_int, [_vp]),
+    "vfps_batch_run": (_int, [_vp, _u32]),
+    "vfps_batch_record": (_int, [_vp, _u32, _int]),
+    "vfps_batch_get_history": (_int, [_vp, _u32, _vp, _vp, _vp]),
+    "vfps_batch_get_mesh": (_int, [_vp, _u32, _vp, _vp, _vp]),
+    "vfps_batch_info": (_int, [_vp, _vp, _vp, _vp]),
+    "vfps_batch_config": (_int, [_vp, _u32, _vp, _u32]),
+    "vfps_batch_timing": (_int, [_vp, _u32, _vp]),
+    "vfps_batch_autotune": (_int, [_vp, _u32, _vp, _u32, _vp]),
+    "vfps_toeplitz_kernels": (_int, [_u32, _u32, _vp, _vp, _vp, _f32, _vp,
+                                     _vp]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

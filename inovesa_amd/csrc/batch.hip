@@ -1,0 +1,523 @@
+// Many independent meshes of uneven sizes advanced together.
+//
+// Every instance is a fused step with the direct wake (its own meshes,
+// maps, impedance kernels, counters and histories).  The batch launches the
+// step kernel once per step for all instances that share a kernel variant
+// and workgroup size: the workgroups of one launch are the row tiles of all
+// its instances (largest instances first), each reading its instance's
+// descriptor, so instances of 128^2 to 1024^2 cells share the GPU in one
+// grid.  Launches of different variants run as parallel branches.  The
+// steps are captured into a hipGraph and replayed.
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <tuple>
+
+#include "common.hpp"
+#include "step.hpp"
+#include "step_kernel.hpp"
+
+using namespace vfps_impl;
+
+namespace {
+
+struct Group {
+    StepVariant variant{};
+    uint32_t threads = 256;
+    std::vector<uint32_t> members;  // batch indices
+    InstDesc* descs_d = nullptr;
+    uint32_t* blockmap_d = nullptr;
+    uint32_t nblocks = 0;
+    size_t lds = 0;
+    hipStream_t stream = nullptr;   // branch stream (groups after the first)
+    hipEvent_t done = nullptr;
+};
+
+void group_free(Group& g)
+{
+    hipFree(g.descs_d);
+    hipFree(g.blockmap_d);
+    if (g.stream) hipStreamDestroy(g.stream);
+    if (g.done) hipEventDestroy(g.done);
+    g = Group{};
+}
+
+}  // namespace
+
+struct vfps_batch {
+    vfps_dev* dev = nullptr;
+    std::vector<vfps_step*> members;
+    std::vector<Group> groups;
+    hipEvent_t fork = nullptr;
+    uint32_t parity = 0;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    uint32_t graph_steps = 0;
+};
+
+namespace {
+
+int drop_graph(vfps_batch* b)
+{
+    if (b->graph) {
+        VFPS_HIP(hipGraphExecDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    if (b->graph_def) {
+        VFPS_HIP(hipGraphDestroy(b->graph_def));
+        b->graph_def = nullptr;
+    }
+    return VFPS_OK;
+}
+
+void free_groups(vfps_batch* b)
+{
+    for (auto& g : b->groups) group_free(g);
+    b->groups.clear();
+}
+
+bool same_variant(const StepVariant& a, const StepVariant& c)
+{
+    return a.scheme == c.scheme && a.it == c.it && a.ipfp == c.ipfp &&
+           a.direct == c.direct && a.clamp == c.clamp;
+}
+
+// one group's device descriptors and block map
+int build_group(vfps_batch* b, Group& g)
+{
+    // largest instances first, so their tiles are dealt out first
+    std::vector<uint32_t> order(g.members.size());
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        const auto* a = b->members[g.members[x]];
+        const auto* c = b->members[g.members[y]];
+        return (uint64_t)a->desc.n * a->desc.n > (uint64_t)c->desc.n * c->desc.n;
+    });
+    std::vector<uint32_t> map;
+    g.lds = 0;
+    for (uint32_t k : order) {
+        const vfps_step* s = b->members[g.members[k]];
+        for (uint32_t t = 0; t < s->tiles(); t++) {
+            map.push_back((k << 16) | t);
+        }
+        g.lds = std::max(g.lds, step_lds_bytes(s->variant, s->desc.n, s->T,
+                                               s->stride));
+    }
+    while (map.size() % 8u != 0u) {
+        map.push_back(STEP_NO_TILE);
+    }
+    VFPS_CHECK_ARG(g.lds <= b->dev->lds_max,
+                   "batch: a tile exceeds the LDS of one CU");
+    g.nblocks = (uint32_t)map.size();
+    VFPS_HIP(hipMalloc(&g.blockmap_d, sizeof(uint32_t) * map.size()));
+    VFPS_HIP(hipMemcpy(g.blockmap_d, map.data(), sizeof(uint32_t) * map.size(),
+                       hipMemcpyHostToDevice));
+    VFPS_HIP(hipMalloc(&g.descs_d, sizeof(InstDesc) * g.members.size()));
+    for (size_t k = 0; k < g.members.size(); k++) {
+        VFPS_HIP(hipMemcpy(g.descs_d + k, b->members[g.members[k]]->inst_d,
+                           sizeof(InstDesc), hipMemcpyDeviceToDevice));
+    }
+    int rc = step_set_lds_limit(g.variant, g.lds);
+    if (rc != VFPS_OK) return rc;
+    VFPS_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
+    return VFPS_OK;
+}
+
+// (re)build the launch groups from the members' current configuration
+int build_groups(vfps_batch* b)
+{
+    int rc = drop_graph(b);
+    if (rc != VFPS_OK) return rc;
+    free_groups(b);
+    for (uint32_t i = 0; i < b->members.size(); i++) {
+        const vfps_step* s = b->members[i];
+        Group* g = nullptr;
+        for (auto& e : b->groups) {
+            if (same_variant(e.variant, s->variant) && e.threads == s->threads) {
+                g = &e;
+                break;
+            }
+        }
+        if (!g) {
+            b->groups.emplace_back();
+            g = &b->groups.back();
+            g->variant = s->variant;
+            g->threads = s->threads;
+        }
+        g->members.push_back(i);
+    }
+    for (size_t k = 0; k < b->groups.size(); k++) {
+        Group& g = b->groups[k];
+        if ((rc = build_group(b, g)) != VFPS_OK) return rc;
+        if (k > 0) {
+            VFPS_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+        }
+    }
+    return VFPS_OK;
+}
+
+// one step of every instance: the first group on the device stream, the
+// others on branch streams forked from and joined back into it
+int launch_step(vfps_batch* b, uint32_t parity)
+{
+    hipStream_t main = b->dev->stream;
+    if (b->groups.size() > 1) {
+        VFPS_HIP(hipEventRecord(b->fork, main));
+    }
+    for (size_t k = 0; k < b->groups.size(); k++) {
+        Group& g = b->groups[k];
+        hipStream_t st = k == 0 ? main : g.stream;
+        if (k > 0) {
+            VFPS_HIP(hipStreamWaitEvent(st, b->fork, 0));
+        }
+        int rc = launch_step_kernel(g.variant, g.descs_d, g.blockmap_d,
+                                    g.nblocks, g.lds, g.threads, parity, st);
+        if (rc != VFPS_OK) return rc;
+        if (k > 0) {
+            VFPS_HIP(hipEventRecord(g.done, st));
+        }
+    }
+    for (size_t k = 1; k < b->groups.size(); k++) {
+        VFPS_HIP(hipStreamWaitEvent(main, b->groups[k].done, 0));
+    }
+    return VFPS_OK;
+}
+
+int capture(vfps_batch* b, uint32_t steps)
+{
+    int rc = drop_graph(b);
+    if (rc != VFPS_OK) return rc;
+    hipStream_t st = b->dev->stream;
+    VFPS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (uint32_t k = 0; k < steps; k++) {
+        rc = launch_step(b, k & 1u);
+        if (rc != VFPS_OK) {
+            hipGraph_t g;
+            hipStreamEndCapture(st, &g);
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+    }
+    VFPS_HIP(hipStreamEndCapture(st, &b->graph_def));
+    VFPS_HIP(hipGraphInstantiate(&b->graph, b->graph_def, nullptr, nullptr, 0));
+    b->graph_steps = steps;
+    return VFPS_OK;
+}
+
+void sync_parity(vfps_batch* b)
+{
+    for (auto* s : b->members) s->parity = b->parity;
+}
+
+void batch_free(vfps_batch* b)
+{
+    if (!b) return;
+    if (b->dev) hipStreamSynchronize(b->dev->stream);
+    drop_graph(b);
+    free_groups(b);
+    if (b->fork) hipEventDestroy(b->fork);
+    for (auto* s : b->members) vfps_step_destroy(s);
+    delete b;
+}
+
+// state of one instance saved around the autotuner's timing runs
+struct Snapshot {
+    float* mesh = nullptr;
+    float* proj = nullptr;
+    uint32_t counter = 0;
+};
+
+}  // namespace
+
+extern "C" {
+
+int vfps_batch_create(vfps_dev* dev, uint32_t ninst,
+                      const vfps_step_desc* descs, const float* const* inits,
+                      vfps_impedance* const* zwake,
+                      vfps_impedance* const* zcsr, vfps_batch** out)
+{
+    VFPS_CHECK_ARG(dev && descs && inits && out && ninst > 0,
+                   "vfps_batch_create: NULL argument or empty batch");
+    VFPS_CHECK_ARG(ninst <= 0xFFFFu, "vfps_batch_create: too many instances");
+    VFPS_TRY_BEGIN
+    auto* b = new vfps_batch();
+    std::unique_ptr<vfps_batch, void (*)(vfps_batch*)> guard(b, batch_free);
+    b->dev = dev;
+    for (uint32_t i = 0; i < ninst; i++) {
+        vfps_step_desc d = descs[i];
+        d.fused = 1;
+        d.wake_method = 2;
+        vfps_step* s = nullptr;
+        int rc = vfps_step_create(dev, &d, inits[i], zwake ? zwake[i] : nullptr,
+                                  zcsr ? zcsr[i] : nullptr, &s);
+        if (rc != VFPS_OK) return rc;
+        b->members.push_back(s);
+        VFPS_CHECK_ARG(s->tiles() <= 0xFFFFu, "vfps_batch_create: too many tiles");
+    }
+    VFPS_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    int rc = build_groups(b);
+    if (rc != VFPS_OK) return rc;
+    VFPS_HIP(hipStreamSynchronize(dev->stream));
+    *out = guard.release();
+    return VFPS_OK;
+    VFPS_TRY_END
+}
+
+int vfps_batch_destroy(vfps_batch* b)
+{
+    batch_free(b);
+    return VFPS_OK;
+}
+
+int vfps_batch_run(vfps_batch* b, uint32_t nsteps)
+{
+    VFPS_CHECK_ARG(b, "vfps_batch_run: NULL batch");
+    hipStream_t st = b->dev->stream;
+    uint32_t left = nsteps;
+    int rc;
+    if ((b->parity & 1u) && left > 0) {
+        if ((rc = launch_step(b, 1)) != VFPS_OK) return rc;
+        b->parity = 0;
+        left--;
+    }
+    const uint32_t G = 64;
+    if (left >= G) {
+        if (!b->graph || b->graph_steps != G) {
+            if ((rc = capture(b, G)) != VFPS_OK) return rc;
+        }
+        while (left >= G) {
+            VFPS_HIP(hipGraphLaunch(b->graph, st));
+            left -= G;
+        }
+    }
+    while (left > 0) {
+        if ((rc = launch_step(b, b->parity)) != VFPS_OK) return rc;
+        b->parity ^= 1u;
+        left--;
+    }
+    VFPS_HIP(hipStreamSynchronize(st));
+    sync_parity(b);
+    for (auto* s : b->members) {
+        uint32_t err = 0;
+        VFPS_HIP(hipMemcpy(&err, s->err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (err) {
+            return fail(VFPS_ERR_INVALID, "batch step: a tap fell outside the "
+                                          "halo rows (internal error)");
+        }
+    }
+    return VFPS_OK;
+}
+
+int vfps_batch_record(vfps_batch* b, uint32_t capacity, int want_profiles)
+{
+    VFPS_CHECK_ARG(b, "vfps_batch_record: NULL batch");
+    for (auto* s : b->members) {
+        int rc = vfps_step_record(s, capacity, want_profiles);
+        if (rc != VFPS_OK) return rc;
+    }
+    return build_groups(b);
+}
+
+int vfps_batch_get_history(vfps_batch* b, uint32_t inst, uint32_t* count,
+                           float* csr_power, float* profiles)
+{
+    VFPS_CHECK_ARG(b && inst < b->members.size(), "vfps_batch_get_history: "
+                                                  "bad instance");
+    return vfps_step_get_history(b->members[inst], count, csr_power, profiles);
+}
+
+int vfps_batch_get_mesh(vfps_batch* b, uint32_t inst, float* mesh,
+                        float* proj, float* wake)
+{
+    VFPS_CHECK_ARG(b && inst < b->members.size(), "vfps_batch_get_mesh: "
+                                                  "bad instance");
+    sync_parity(b);
+    return vfps_step_get_mesh(b->members[inst], mesh, proj, wake);
+}
+
+int vfps_batch_info(vfps_batch* b, uint32_t* ninst, uint32_t* ngroups,
+                    uint64_t* cells)
+{
+    VFPS_CHECK_ARG(b, "vfps_batch_info: NULL batch");
+    uint64_t c = 0;
+    for (auto* s : b->members) c += (uint64_t)s->desc.n * s->desc.n;
+    if (ninst) *ninst = (uint32_t)b->members.size();
+    if (ngroups) *ngroups = (uint32_t)b->groups.size();
+    if (cells) *cells = c;
+    return VFPS_OK;
+}
+
+int vfps_batch_config(vfps_batch* b, uint32_t inst, uint32_t* out,
+                      uint32_t nout)
+{
+    VFPS_CHECK_ARG(b && inst < b->members.size() && out && nout >= 6,
+                   "vfps_batch_config: bad argument");
+    return vfps_step_config(b->members[inst], out, nout);
+}
+
+int vfps_batch_timing(vfps_batch* b, uint32_t nsteps, float* ms_step)
+{
+    VFPS_CHECK_ARG(b && nsteps > 0 && ms_step, "vfps_batch_timing: bad argument");
+    hipStream_t st = b->dev->stream;
+    hipEvent_t e0, e1;
+    VFPS_HIP(hipEventCreate(&e0));
+    VFPS_HIP(hipEventCreate(&e1));
+    double tot = 0;
+    int rc = VFPS_OK;
+    for (uint32_t k = 0; k < nsteps && rc == VFPS_OK; k++) {
+        hipEventRecord(e0, st);
+        rc = launch_step(b, b->parity);
+        hipEventRecord(e1, st);
+        hipEventSynchronize(e1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        tot += ms;
+        b->parity ^= 1u;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    sync_parity(b);
+    if (rc != VFPS_OK) return rc;
+    *ms_step = (float)(tot / nsteps);
+    return VFPS_OK;
+}
+
+// Choose the tile height and workgroup size per mesh size: for each size
+// class, every candidate (T, threads) is timed on that class's instances
+// alone (one launch per step, HIP events); the fastest is kept.  The state
+// of every instance is saved before and restored after, so tuning does not
+// advance the simulation.  table rows: n, T, threads, microseconds per
+// step, chosen (1/0).
+int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
+                        uint32_t cap, uint32_t* nrows)
+{
+    VFPS_CHECK_ARG(b && steps > 0, "vfps_batch_autotune: bad argument");
+    VFPS_TRY_BEGIN
+    hipStream_t st = b->dev->stream;
+    VFPS_HIP(hipStreamSynchronize(st));
+    sync_parity(b);
+    const uint32_t par = b->parity;
+    // save the state
+    std::vector<Snapshot> snap(b->members.size());
+    for (size_t i = 0; i < b->members.size(); i++) {
+        vfps_step* s = b->members[i];
+        const size_t cells = (size_t)s->desc.n * s->desc.n;
+        VFPS_HIP(hipMalloc(&snap[i].mesh, sizeof(float) * cells));
+        VFPS_HIP(hipMalloc(&snap[i].proj, sizeof(float) * s->desc.n));
+        const vfps_mesh* cur = par ? s->t2 : s->t1;
+        VFPS_HIP(hipMemcpy(snap[i].mesh, cur->data, sizeof(float) * cells,
+                           hipMemcpyDeviceToDevice));
+        VFPS_HIP(hipMemcpy(snap[i].proj, cur->proj, sizeof(float) * s->desc.n,
+                           hipMemcpyDeviceToDevice));
+        VFPS_HIP(hipMemcpy(&snap[i].counter, s->counter, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost));
+    }
+    std::map<uint32_t, std::vector<uint32_t>> classes;
+    for (uint32_t i = 0; i < b->members.size(); i++) {
+        classes[b->members[i]->desc.n].push_back(i);
+    }
+    const uint32_t even = steps + (steps & 1u);
+    uint32_t rows = 0;
+    int rc = VFPS_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    VFPS_HIP(hipEventCreate(&e0));
+    VFPS_HIP(hipEventCreate(&e1));
+    for (auto& cl : classes) {
+        const uint32_t n = cl.first;
+        float best = 1e30f;
+        uint32_t bestT = b->members[cl.second[0]]->T;
+        uint32_t bestThr = b->members[cl.second[0]]->threads;
+        const uint32_t first_row = rows;
+        for (uint32_t thr : {128u, 256u, 512u}) {
+            for (uint32_t T : {1u, 2u, 4u, 8u}) {
+                if (ceil_div(n, thr) > (uint32_t)STEP_MAX_COLS || T > n) continue;
+                bool ok = true;
+                for (uint32_t i : cl.second) {
+                    if (step_retile(b->members[i], T, thr) != VFPS_OK) {
+                        ok = false;
+                        break;
+                    }
+                }
+                if (!ok) continue;
+                // a batch of this class only
+                vfps_batch sub;
+                sub.dev = b->dev;
+                sub.fork = b->fork;
+                for (uint32_t i : cl.second) sub.members.push_back(b->members[i]);
+                rc = build_groups(&sub);
+                float ms = 0;
+                if (rc == VFPS_OK) {
+                    // warm up, then time
+                    for (uint32_t k = 0; k < 2 && rc == VFPS_OK; k++) {
+                        rc = launch_step(&sub, (par + k) & 1u);
+                    }
+                    hipEventRecord(e0, st);
+                    for (uint32_t k = 0; k < even && rc == VFPS_OK; k++) {
+                        rc = launch_step(&sub, (par + k) & 1u);
+                    }
+                    hipEventRecord(e1, st);
+                    hipEventSynchronize(e1);
+                    hipEventElapsedTime(&ms, e0, e1);
+                }
+                free_groups(&sub);
+                sub.fork = nullptr;
+                sub.members.clear();
+                if (rc != VFPS_OK) break;
+                const float us = 1e3f * ms / (float)even;
+                if (table && rows < cap) {
+                    float* r = table + 5 * (size_t)rows;
+                    r[0] = (float)n;
+                    r[1] = (float)T;
+                    r[2] = (float)thr;
+                    r[3] = us;
+                    r[4] = 0.0f;
+                }
+                rows++;
+                if (us < best) {
+                    best = us;
+                    bestT = T;
+                    bestThr = thr;
+                }
+            }
+            if (rc != VFPS_OK) break;
+        }
+        if (rc != VFPS_OK) break;
+        for (uint32_t i : cl.second) {
+            if ((rc = step_retile(b->members[i], bestT, bestThr)) != VFPS_OK) break;
+        }
+        if (table) {
+            for (uint32_t r = first_row; r < rows && r < cap; r++) {
+                float* row = table + 5 * (size_t)r;
+                if ((uint32_t)row[1] == bestT && (uint32_t)row[2] == bestThr) {
+                    row[4] = 1.0f;
+                }
+            }
+        }
+        if (rc != VFPS_OK) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    // restore the state
+    hipStreamSynchronize(st);
+    for (size_t i = 0; i < b->members.size(); i++) {
+        vfps_step* s = b->members[i];
+        const size_t cells = (size_t)s->desc.n * s->desc.n;
+        const vfps_mesh* cur = par ? s->t2 : s->t1;
+        hipMemcpy(cur->data, snap[i].mesh, sizeof(float) * cells,
+                  hipMemcpyDeviceToDevice);
+        hipMemcpy(cur->proj, snap[i].proj, sizeof(float) * s->desc.n,
+                  hipMemcpyDeviceToDevice);
+        hipMemcpy(s->counter, &snap[i].counter, sizeof(uint32_t),
+                  hipMemcpyHostToDevice);
+        hipFree(snap[i].mesh);
+        hipFree(snap[i].proj);
+        s->parity = par;
+    }
+    if (rc != VFPS_OK) return rc;
+    if (nrows) *nrows = rows;
+    return build_groups(b);
+    VFPS_TRY_END
+}
+
+}  // extern "C"

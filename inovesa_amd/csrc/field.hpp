@@ -12,6 +12,7 @@ struct FieldParams {
     const uint32_t* buckets;
     float scaling;            // wake scaling already divided by nmax
     const float* proj;        // [nb][n] x projection
+    const float* proj_alt;    // x projection read at odd parity (fused step)
     const float* ws;          // Simpson weights (for the FP64 integral)
     const float2* Z;          // wake impedance [2M]
     const double2* tw_m;      // e^{-2 pi i t/M}

@@ -24,391 +24,60 @@
 #include "field_body.hpp"
 #include "reduce.hpp"
 #include "step.hpp"
+#include "step_kernel.hpp"
 
 namespace vfps_impl {
 
-// ---- fused maps kernel -------------------------------------------------------
+// ---- fused step kernel ---------------------------------------------------------
 
-template <int IT>
-__device__ __forceinline__ void row_kick(const float* __restrict__ src,
-                                         float* __restrict__ dst, uint32_t n,
-                                         float off, uint32_t lane)
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP>
+static const void* step_fn()
 {
-    uint32_t idx[IT];
-    float w[IT];
-    kick_taps<IT>(off, n, idx, w);
-    const int32_t half = (int32_t)(n / 2);
-    for (uint32_t y = lane; y < n; y += 64) {
-        float v = 0.0f;
-#pragma unroll
-        for (int j = 0; j < IT; j++) {
-            const uint32_t ys = (uint32_t)((int32_t)(y + idx[j]) - half);
-            if (ys < n) {
-                v = v + src[ys] * w[j];
-            }
-        }
-        dst[y] = v;
-    }
+    return reinterpret_cast<const void*>(&step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP>);
 }
 
-// most mesh columns one thread handles (n <= 1024 with >= 256 threads)
-constexpr int MAPS_MAX_COLS = 4;
-// float4 loads one thread keeps in flight while filling the tile
-constexpr int MAPS_LOAD_BATCH = 8;
-
-#define VFPS_STAMP(k)                                                         \
-    do {                                                                      \
-        if (d.tstamp && blockIdx.x == 0) {                                    \
-            __syncthreads();                                                  \
-            if (threadIdx.x == 0) d.tstamp[k] = wall_clock64();               \
-        }                                                                     \
-    } while (0)
-
-template <int IT, int IPFP, bool WAKE, int SCHEME, bool CLAMP>
-__global__ void __launch_bounds__(512)
-maps_kernel(const InstDesc* __restrict__ descs, uint32_t tiles_per_inst,
-            uint32_t T, uint32_t hmax, uint32_t parity)
+template <int IT, int IPFP, bool DIRECT>
+static const void* step_fn_scheme(const StepVariant& v)
 {
-    extern __shared__ __align__(16) float lds[];
-    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
-    // so give each XCD a contiguous run of tiles (neighbouring tiles share
-    // halo rows, which then hit in that XCD's L2).
-    const uint32_t nblocks = gridDim.x;
-    uint32_t logical = blockIdx.x;
-    if ((nblocks & 7u) == 0u) {
-        logical = (blockIdx.x & 7u) * (nblocks >> 3) + (blockIdx.x >> 3);
-    }
-    const InstDesc& d = descs[logical / tiles_per_inst];
-    const uint32_t tile = logical % tiles_per_inst;
-    const uint32_t n = d.n;
-    const int x0 = (int)(tile * T);
-    if (x0 >= (int)n) {
-        return;
-    }
-    VFPS_STAMP(0);
-    const int rows_out = min((int)T, (int)n - x0);
-    const int hlo = (int)d.hlo;
-    const int rbase = x0 - hlo;
-    const int R = rows_out + hlo + (int)d.hhi;
-    const int stride = (int)(T + 2 * hmax);  // rows per LDS buffer
-    const uint32_t tstride = n + 4;           // padded row of the output tile
-    float* bufA = lds;
-    float* bufB = bufA + (size_t)stride * n;
-    float* tile_out = bufB + (size_t)stride * n;  // [T][n+4] products
-    float* offw = tile_out + (size_t)T * tstride;  // [stride] wake offsets
-    float* offr = offw + stride;                   // [stride] RF offsets
-    const float* in = parity ? d.mesh1 : d.mesh0;
-    float* out = parity ? d.mesh0 : d.mesh1;
-    const uint32_t tid = threadIdx.x, bs = blockDim.x;
-    const uint32_t lane = tid & 63u, wave = tid >> 6, nw = bs >> 6;
-    const int32_t half = (int32_t)(n / 2);
-
-    // phase 1: every global read the block needs, issued together
-    //  - per-row kick offsets of the tile's rows
-    if ((int)tid < R) {
-        const int r = rbase + (int)tid;
-        if (r >= 0 && r < (int)n) {
-            if (WAKE) offw[tid] = d.wake_off[r];
-            if (SCHEME == 0) offr[tid] = d.rf_off[r];
-        }
-    }
-    //  - per-column tables (drift offset, Fokker-Planck stencil, Simpson
-    //    weight) of this thread's columns, kept in registers
-    const int ncols = (int)((n + bs - 1) / bs);
-    float col_drift[MAPS_MAX_COLS], col_ws[MAPS_MAX_COLS];
-    uint32_t col_fi[MAPS_MAX_COLS][IPFP > 0 ? IPFP : 1];
-    float col_fw[MAPS_MAX_COLS][IPFP > 0 ? IPFP : 1];
-#pragma unroll
-    for (int c = 0; c < MAPS_MAX_COLS; c++) {
-        const uint32_t y = tid + (uint32_t)c * bs;
-        if (c < ncols && y < n) {
-            if (SCHEME == 0) col_drift[c] = d.drift_off[y];
-            col_ws[c] = d.ws[y];
-#pragma unroll
-            for (int k = 0; k < IPFP; k++) {
-                col_fi[c][k] = d.fp_i[(size_t)k * n + y];
-                col_fw[c][k] = d.fp_w[(size_t)k * n + y];
-            }
-        }
-    }
-    //  - rows [rbase, rbase+R) of the input, MAPS_LOAD_BATCH float4 loads
-    //    in flight per thread
-    if ((n & 3u) == 0u) {
-        const uint32_t n4 = n >> 2;
-        const float4* in4 = reinterpret_cast<const float4*>(in);
-        float4* a4 = reinterpret_cast<float4*>(bufA);
-        const uint32_t total = (uint32_t)R * n4;
-        for (uint32_t base = 0; base < total; base += bs * MAPS_LOAD_BATCH) {
-            float4 v[MAPS_LOAD_BATCH];
-#pragma unroll
-            for (int b = 0; b < MAPS_LOAD_BATCH; b++) {
-                const uint32_t i = base + tid + (uint32_t)b * bs;
-                const int r = rbase + (int)(i / n4);
-                if (i < total && r >= 0 && r < (int)n) {
-                    v[b] = in4[(size_t)r * n4 + (i % n4)];
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < MAPS_LOAD_BATCH; b++) {
-                const uint32_t i = base + tid + (uint32_t)b * bs;
-                const int r = rbase + (int)(i / n4);
-                if (i < total && r >= 0 && r < (int)n) {
-                    a4[i] = v[b];
-                }
-            }
-        }
-    } else {
-        for (uint32_t i = tid; i < (uint32_t)R * n; i += bs) {
-            const int r = rbase + (int)(i / n);
-            if (r >= 0 && r < (int)n) {
-                bufA[i] = in[(size_t)r * n + (i % n)];
-            }
-        }
-    }
-    __syncthreads();
-    VFPS_STAMP(1);
-    float* cur = bufA;
-    float* oth = bufB;
-    // phase 2: wake kick along y (WakePotentialMap::apply), row by row
-    if (WAKE) {
-        for (int row = (int)wave; row < R; row += (int)nw) {
-            const int r = rbase + row;
-            if (r < 0 || r >= (int)n) {
-                continue;
-            }
-            row_kick<IT>(cur + (size_t)row * n, oth + (size_t)row * n, n,
-                         offw[row], lane);
-        }
-        __syncthreads();
-        float* t = cur;
-        cur = oth;
-        oth = t;
-    }
-    VFPS_STAMP(2);
-    if constexpr (SCHEME == 0) {
-        // phase 3: RF kick along y (RFKickMap::apply)
-        for (int row = (int)wave; row < R; row += (int)nw) {
-            const int r = rbase + row;
-            if (r < 0 || r >= (int)n) {
-                continue;
-            }
-            row_kick<IT>(cur + (size_t)row * n, oth + (size_t)row * n, n,
-                         offr[row], lane);
-        }
-        __syncthreads();
-        float* t = cur;
-        cur = oth;
-        oth = t;
-        VFPS_STAMP(3);
-        // phase 4: drift along x (DriftMap::apply), one column per thread
-#pragma unroll
-        for (int c = 0; c < MAPS_MAX_COLS; c++) {
-            const uint32_t y = tid + (uint32_t)c * bs;
-            if (c >= ncols || y >= n) {
-                continue;
-            }
-            uint32_t idx[IT];
-            float w[IT];
-            kick_taps<IT>(col_drift[c], n, idx, w);
-            for (int orow = 0; orow < rows_out; orow++) {
-                const uint32_t x = (uint32_t)(x0 + orow);
-                float v = 0.0f;
-#pragma unroll
-                for (int j = 0; j < IT; j++) {
-                    const uint32_t xs = (uint32_t)((int32_t)(x + idx[j]) - half);
-                    if (xs < n) {
-                        int lr = (int)xs - rbase;
-                        if (lr < 0 || lr >= R) {
-                            atomicOr(d.err, 1u);
-                            lr = lr < 0 ? 0 : R - 1;
-                        }
-                        v = v + cur[(size_t)lr * n + y] * w[j];
-                    }
-                }
-                oth[(size_t)orow * n + y] = v;
-            }
-        }
-    } else {
-        // phase 3: rotation (RotationMap::apply).  Taps out of range read
-        // sample 0 of the (wake-kicked) input, as the reference does.
-        float s00;
-        if (rbase <= 0) {
-            s00 = cur[(size_t)(-rbase) * n];
-        } else if (WAKE) {
-            uint32_t idx[IT];
-            float w[IT];
-            kick_taps<IT>(d.wake_off[0], n, idx, w);
-            float v = 0.0f;
-#pragma unroll
-            for (int j = 0; j < IT; j++) {
-                const uint32_t ys = (uint32_t)((int32_t)idx[j] - half);
-                if (ys < n) {
-                    v = v + in[ys] * w[j];
-                }
-            }
-            s00 = v;
-        } else {
-            s00 = in[0];
-        }
-        VFPS_STAMP(3);
-        const RotGeom g = d.rot;
-        const float* src = cur;
-        for (uint32_t i = tid; i < (uint32_t)rows_out * n; i += bs) {
-            const uint32_t orow = i / n, y0 = i % n;
-            const float v = rotate_cell_src<IT, CLAMP>(
-                n, g, (uint32_t)x0 + orow, y0,
-                [&](bool ok, uint32_t i0, uint32_t j0) -> float {
-                    if (!ok) {
-                        return s00;
-                    }
-                    int lr = (int)i0 - rbase;
-                    if (lr < 0 || lr >= R) {
-                        atomicOr(d.err, 2u);
-                        lr = lr < 0 ? 0 : R - 1;
-                    }
-                    return src[(size_t)lr * n + j0];
-                });
-            oth[i] = v;
-        }
-    }
-    __syncthreads();
-    VFPS_STAMP(4);
-    // phase 5: Fokker-Planck along y (FokkerPlanckMap::apply), the output
-    // store and the products mesh*weight of the x projection, one column
-    // per thread with the column's stencil in registers
-#pragma unroll
-    for (int c = 0; c < MAPS_MAX_COLS; c++) {
-        const uint32_t y = tid + (uint32_t)c * bs;
-        if (c >= ncols || y >= n) {
-            continue;
-        }
-        for (int orow = 0; orow < rows_out; orow++) {
-            const float* src = oth + (size_t)orow * n;
-            float v;
-            if constexpr (IPFP == 0) {
-                v = src[y];
-            } else {
-                v = 0.0f;
-#pragma unroll
-                for (int k = 0; k < IPFP; k++) {
-                    v = v + src[col_fi[c][k]] * col_fw[c][k];
-                }
-            }
-            out[(size_t)(x0 + orow) * n + y] = v;
-            tile_out[(size_t)orow * tstride + y] = v * col_ws[c];
-        }
-    }
-    uint32_t slot = 0xFFFFFFFFu;
-    if (d.prof_hist) {
-        const uint32_t cnt = *d.counter;
-        if (cnt >= 1 && cnt - 1 < d.hist_cap) {
-            slot = cnt - 1;
-        }
-    }
-    __syncthreads();
-    VFPS_STAMP(5);
-    // phase 6: the x projection (PhaseSpace::updateXProjection)
-    if (d.seq_proj) {
-        // the reference's std::inner_product order: y = 0..n-1, one thread
-        // per row, loads batched ahead of the dependent adds
-        if ((int)tid < rows_out) {
-            const float* r = tile_out + (size_t)tid * tstride;
-            float acc = 0.0f;
-            uint32_t y = 0;
-            if ((n & 3u) == 0u) {
-                const float4* r4 = reinterpret_cast<const float4*>(r);
-                for (; y + 16 <= n; y += 16) {
-                    float4 q[4];
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        q[k] = r4[(y >> 2) + k];
-                    }
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        acc = acc + q[k].x;
-                        acc = acc + q[k].y;
-                        acc = acc + q[k].z;
-                        acc = acc + q[k].w;
-                    }
-                }
-            }
-            for (; y < n; y++) {
-                acc = acc + r[y];
-            }
-            const uint32_t x = (uint32_t)x0 + tid;
-            d.proj[x] = acc;
-            if (slot != 0xFFFFFFFFu) {
-                d.prof_hist[(size_t)slot * n + x] = acc;
-            }
-        }
-    } else {
-        for (int orow = (int)wave; orow < rows_out; orow += (int)nw) {
-            const float* r = tile_out + (size_t)orow * tstride;
-            float acc = 0.0f;
-            for (uint32_t y = lane; y < n; y += 64) {
-                acc = acc + r[y];
-            }
-            acc = wave_sum(acc);
-            if (lane == 0) {
-                const uint32_t x = (uint32_t)(x0 + orow);
-                d.proj[x] = acc;
-                if (slot != 0xFFFFFFFFu) {
-                    d.prof_hist[(size_t)slot * n + x] = acc;
-                }
-            }
-        }
-    }
-    VFPS_STAMP(6);
+    if (v.scheme == 0) return step_fn<IT, IPFP, DIRECT, 0, false>();
+    if (v.clamp) return step_fn<IT, IPFP, DIRECT, 1, true>();
+    return step_fn<IT, IPFP, DIRECT, 1, false>();
 }
 
-template <int IT, int IPFP, bool WAKE, int SCHEME, bool CLAMP>
-static const void* maps_fn()
+template <int IT, bool DIRECT>
+static const void* step_fn_fp(const StepVariant& v)
 {
-    return reinterpret_cast<const void*>(&maps_kernel<IT, IPFP, WAKE, SCHEME, CLAMP>);
-}
-
-// resolve a variant to its kernel (cubic and linear interpolation)
-static const void* maps_kernel_for(const MapsVariant& v)
-{
-#define VFPS_SEL(IT, IP)                                                      \
-    if (v.scheme == 0) {                                                      \
-        return v.wake ? maps_fn<IT, IP, true, 0, false>()                     \
-                      : maps_fn<IT, IP, false, 0, false>();                   \
-    }                                                                         \
-    if (v.clamp) {                                                            \
-        return v.wake ? maps_fn<IT, IP, true, 1, true>()                      \
-                      : maps_fn<IT, IP, false, 1, true>();                    \
-    }                                                                         \
-    return v.wake ? maps_fn<IT, IP, true, 1, false>()                         \
-                  : maps_fn<IT, IP, false, 1, false>();
-    if (v.it == 4) {
-        switch (v.ipfp) {
-        case 0: { VFPS_SEL(4, 0) }
-        case 3: { VFPS_SEL(4, 3) }
-        case 4: { VFPS_SEL(4, 4) }
-        case 5: { VFPS_SEL(4, 5) }
-        }
-    } else if (v.it == 2 && !v.clamp) {
-        switch (v.ipfp) {
-        case 0: { VFPS_SEL(2, 0) }
-        case 3: { VFPS_SEL(2, 3) }
-        case 4: { VFPS_SEL(2, 4) }
-        case 5: { VFPS_SEL(2, 5) }
-        }
+    switch (v.ipfp) {
+    case 0: return step_fn_scheme<IT, 0, DIRECT>(v);
+    case 3: return step_fn_scheme<IT, 3, DIRECT>(v);
+    case 4: return step_fn_scheme<IT, 4, DIRECT>(v);
+    case 5: return step_fn_scheme<IT, 5, DIRECT>(v);
     }
-#undef VFPS_SEL
     return nullptr;
 }
 
-size_t fused_lds_bytes(uint32_t n, uint32_t T, uint32_t hmax)
+// resolve a variant to its kernel (cubic and linear interpolation; the
+// clamp needs cubic interpolation)
+const void* step_kernel_for(const StepVariant& v)
 {
-    const size_t stride = T + 2 * hmax;
-    return sizeof(float) * (2 * stride * n + (size_t)T * (n + 4) + 2 * stride);
+    if (v.it == 4) {
+        return v.direct ? step_fn_fp<4, true>(v) : step_fn_fp<4, false>(v);
+    }
+    if (v.it == 2 && !v.clamp) {
+        return v.direct ? step_fn_fp<2, true>(v) : step_fn_fp<2, false>(v);
+    }
+    return nullptr;
 }
 
-int fused_set_lds_limit(const MapsVariant& v, size_t bytes)
+size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
+                      uint32_t stride)
 {
-    const void* fn = maps_kernel_for(v);
+    return sizeof(float) * step_lds(n, T, stride, v.direct, (uint32_t)v.it).words;
+}
+
+int step_set_lds_limit(const StepVariant& v, size_t bytes)
+{
+    const void* fn = step_kernel_for(v);
     if (!fn) {
         return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
                                           "interpolation/stencil combination");
@@ -421,36 +90,95 @@ int fused_set_lds_limit(const MapsVariant& v, size_t bytes)
     return VFPS_OK;
 }
 
-int launch_fused_maps(const MapsVariant& v, const InstDesc* descs_d,
-                      uint32_t ninst, uint32_t n, uint32_t T, uint32_t threads,
-                      uint32_t hmax, uint32_t parity, hipStream_t st)
+int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
+                       const uint32_t* blockmap_d, uint32_t nblocks,
+                       size_t lds_bytes, uint32_t threads, uint32_t parity,
+                       hipStream_t st)
 {
-    const void* fn = maps_kernel_for(v);
+    const void* fn = step_kernel_for(v);
     if (!fn) {
         return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
                                           "interpolation/stencil combination");
     }
-    uint32_t tiles = ceil_div(n, T);
-    void* args[] = {(void*)&descs_d, (void*)&tiles, (void*)&T, (void*)&hmax,
-                    (void*)&parity};
-    VFPS_HIP(hipLaunchKernel(fn, dim3(tiles * ninst), dim3(threads), args,
-                             fused_lds_bytes(n, T, hmax), st));
+    void* args[] = {(void*)&descs_d, (void*)&blockmap_d, (void*)&parity};
+    VFPS_HIP(hipLaunchKernel(fn, dim3(nblocks), dim3(threads), args, lds_bytes,
+                             st));
     return VFPS_OK;
+}
+
+// Toeplitz kernels of the direct wake and CSR forms, in FP64 from the
+// float tables.  With N = nmax = 2M and theta = 2 pi k d / N:
+//   g[d] = Re Z_0 + 2 sum_{k=1}^{M-1} (Re Z_k cos theta - Im Z_k sin theta)
+// is C2R(Z) with the Nyquist bin zero and Im Z_0 dropped, so that
+//   C2R(Z * R2C(pad(p)))[x] = sum_x' g[x - x'] p[x'];
+//   h[d] = sum_{k=0}^{M} c_k cos theta,  c_k = fdelta renorm_k Re Zc_k,
+// gives the CSR power sum_k c_k |F_k|^2 = sum_{x,x'} p[x] h[x - x'] p[x'].
+// Tables hold d = -(n-1)..(n-1) at index d + n - 1.
+void toeplitz_tables(uint32_t n, uint32_t M, const float2* Z,
+                     const float2* Zc, const float* renorm, float fdelta,
+                     std::vector<double>* g, std::vector<double>* h)
+{
+    const uint64_t N = 2ull * M;
+    std::vector<double> cs(N), sn(N);
+    for (uint64_t j = 0; j < N; j++) {
+        const double a = 2.0 * M_PI * (double)j / (double)N;
+        cs[j] = std::cos(a);
+        sn[j] = std::sin(a);
+    }
+    const int64_t nd = 2 * (int64_t)n - 1;
+    if (g) {
+        g->assign((size_t)nd, 0.0);
+        for (int64_t i = 0; i < nd; i++) {
+            const int64_t dd = i - (int64_t)(n - 1);
+            const uint64_t step = (uint64_t)((dd % (int64_t)N + (int64_t)N) % (int64_t)N);
+            double acc = 0.0;
+            uint64_t j = 0;
+            for (uint32_t k = 1; k < M; k++) {
+                j += step;
+                if (j >= N) j -= N;
+                acc += (double)Z[k].x * cs[j] - (double)Z[k].y * sn[j];
+            }
+            (*g)[(size_t)i] = (double)Z[0].x + 2.0 * acc;
+        }
+    }
+    if (h) {
+        std::vector<double> c(M + 1);
+        for (uint32_t k = 0; k <= M; k++) {
+            c[k] = (double)fdelta * (double)renorm[k] * (double)Zc[k].x;
+        }
+        h->assign((size_t)nd, 0.0);
+        for (int64_t i = 0; i < nd; i++) {
+            const int64_t dd = i - (int64_t)(n - 1);
+            const uint64_t step = (uint64_t)((dd % (int64_t)N + (int64_t)N) % (int64_t)N);
+            double acc = c[0];
+            uint64_t j = 0;
+            for (uint32_t k = 1; k <= M; k++) {
+                j += step;
+                if (j >= N) j -= N;
+                acc += c[k] * cs[j];
+            }
+            (*h)[(size_t)i] = acc;
+        }
+    }
 }
 
 // ---- fields kernel (one workgroup per instance) ------------------------------
 
 template <bool WAKE, bool CSR>
 __global__ void __launch_bounds__(1024)
-fields_kernel(const FieldParams* __restrict__ params)
+fields_kernel(const FieldParams* __restrict__ params, uint32_t parity)
 {
     extern __shared__ double2 flds[];
-    const FieldParams p = params[blockIdx.x];
+    FieldParams p = params[blockIdx.x];
+    if (parity && p.proj_alt) {
+        p.proj = p.proj_alt;
+    }
     field_body<WAKE, CSR>(p, 0, flds);
 }
 
 int launch_fields(bool wake, bool csr, const FieldParams* params_d,
-                  uint32_t ninst, uint32_t Mmax, hipStream_t st)
+                  uint32_t ninst, uint32_t Mmax, uint32_t parity,
+                  hipStream_t st)
 {
     const size_t lds = field_lds_bytes(Mmax);
     const void* fn;
@@ -468,7 +196,7 @@ int launch_fields(bool wake, bool csr, const FieldParams* params_d,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds));
     }
-    void* args[] = {(void*)&params_d};
+    void* args[] = {(void*)&params_d, (void*)&parity};
     VFPS_HIP(hipLaunchKernel(fn, dim3(ninst), dim3(FFT_THREADS), args, lds, st));
     return VFPS_OK;
 }
@@ -540,34 +268,19 @@ __global__ void record_kernel(uint32_t n, const float* proj, float* hist,
 
 using namespace vfps_impl;
 
-struct vfps_step {
-    vfps_dev* dev = nullptr;
-    vfps_step_desc desc{};
-    vfps_mesh *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
-    vfps_map *wm = nullptr, *rfm = nullptr, *drm = nullptr, *rot = nullptr,
-             *fpm = nullptr;
-    vfps_field* wfield = nullptr;
-    vfps_impedance* zwake = nullptr;
-    vfps_impedance* zcsr = nullptr;
-    float* csr_renorm = nullptr;
-    double2* csr_tw = nullptr;
-    float csr_fdelta = 0;
-    double* filling = nullptr;  // [2]
-    uint32_t* counter = nullptr;
-    uint32_t* err = nullptr;
-    float* csr_hist = nullptr;
-    float* prof_hist = nullptr;
-    uint32_t hist_cap = 0;
-    InstDesc* inst_d = nullptr;
-    FieldParams* fparams_d = nullptr;
-    MapsVariant variant{};
-    uint32_t T = 8, threads = 256, hmax = 0, M = 0;
-    bool fused = true;
-    uint32_t parity = 0;  // fused path: buffer holding the state (0: t1)
-    hipGraphExec_t graph = nullptr;
-    hipGraph_t graph_def = nullptr;
-    uint32_t graph_steps = 0;
-};
+// the fused step's kernels for one step at the given parity
+static int step_launch_fused(vfps_step* s, uint32_t parity, hipStream_t st)
+{
+    if (!s->direct) {
+        int rc = launch_fields(s->desc.use_wake != 0, s->desc.use_csr != 0,
+                               s->fparams_d, 1, s->M, parity, st);
+        if (rc != VFPS_OK) return rc;
+    }
+    return launch_step_kernel(s->variant, s->inst_d, nullptr, s->tiles(),
+                              step_lds_bytes(s->variant, s->desc.n, s->T,
+                                             s->stride),
+                              s->threads, parity, st);
+}
 
 static int step_launch_one(vfps_step* s, uint32_t parity)
 {
@@ -575,13 +288,10 @@ static int step_launch_one(vfps_step* s, uint32_t parity)
     const bool wake = s->desc.use_wake != 0;
     const bool csr = s->desc.use_csr != 0;
     if (s->fused) {
-        int rc = launch_fields(wake, csr, s->fparams_d, 1, s->M, st);
-        if (rc != VFPS_OK) return rc;
-        return launch_fused_maps(s->variant, s->inst_d, 1, s->desc.n, s->T,
-                                 s->threads, s->hmax, parity, st);
+        return step_launch_fused(s, parity, st);
     }
     // per-map path, the loop body of src/main.cpp:948-1053
-    int rc = launch_fields(wake, csr, s->fparams_d, 1, s->M, st);
+    int rc = launch_fields(wake, csr, s->fparams_d, 1, s->M, 0, st);
     if (rc != VFPS_OK) return rc;
     vfps_mesh *t1 = s->t1, *t2 = s->t2, *t3 = s->t3;
     if (wake) {
@@ -636,6 +346,11 @@ static void step_free(vfps_step* s)
     hipFree(s->err);
     hipFree(s->csr_hist);
     hipFree(s->prof_hist);
+    hipFree(s->wake_g);
+    hipFree(s->csr_h);
+    hipFree(s->csr_part);
+    hipFree(s->ticket);
+    hipFree(s->csr_power);
     hipFree(s->inst_d);
     hipFree(s->fparams_d);
     delete s;
@@ -647,17 +362,30 @@ static int step_upload_desc(vfps_step* s)
     vfps_mesh* t1 = s->t1;
     InstDesc d{};
     d.n = s->desc.n;
-    d.mesh0 = t1->data;
-    d.mesh1 = s->t2->data;
-    d.proj = t1->proj;
+    d.tiles = s->tiles();
+    d.mesh[0] = t1->data;
+    d.mesh[1] = s->t2->data;
+    d.proj[0] = t1->proj;
+    d.proj[1] = s->t2->proj;
     d.ws = t1->ws;
+    d.use_wake = s->desc.use_wake ? 1 : 0;
+    d.use_csr = s->desc.use_csr ? 1 : 0;
     d.wake_off = s->desc.use_wake ? s->wfield->wake : nullptr;
+    d.wake_out = s->desc.use_wake ? s->wfield->wake : nullptr;
+    d.wake_g = s->wake_g;
+    d.wake_scale = s->wfield ? s->wfield->wakescaling : 0.0f;
+    d.csr_h = s->csr_h;
+    d.csr_part = s->csr_part;
+    d.ticket = s->ticket;
+    d.csr_power = s->csr_power;
+    d.filling = s->filling;
     d.rf_off = s->rfm ? s->rfm->offset : nullptr;
     d.drift_off = s->drm ? s->drm->offset : nullptr;
     d.fp_w = s->fpm ? s->fpm->fp_w : nullptr;
     d.fp_i = s->fpm ? s->fpm->fp_i : nullptr;
     if (s->rot) d.rot = rot_geom(s->rot);
     d.prof_hist = s->prof_hist;
+    d.csr_hist = s->csr_hist;
     d.hist_cap = s->hist_cap;
     d.counter = s->counter;
     d.err = s->err;
@@ -673,7 +401,10 @@ static int step_upload_desc(vfps_step* s)
     }
     d.hlo = hlo;
     d.hhi = hhi;
-    s->hmax = std::max(hlo, hhi);
+    s->hmax = hlo + hhi;
+    s->stride = s->T + hlo + hhi;
+    d.T = s->T;
+    d.stride = s->stride;
     VFPS_HIP(hipMemcpy(s->inst_d, &d, sizeof(InstDesc), hipMemcpyHostToDevice));
 
     FieldParams p{};
@@ -682,11 +413,13 @@ static int step_upload_desc(vfps_step* s)
     p.M = s->M;
     p.spacing = s->desc.spacing_bins;
     p.proj = t1->proj;
+    p.proj_alt = s->fused ? s->t2->proj : nullptr;
     p.ws = t1->ws;
     p.filling = s->filling;
     p.counter = s->counter;
     p.csr_hist = s->csr_hist;
     p.hist_cap = s->hist_cap;
+    p.power = s->csr_power;
     if (s->wfield) {
         p.buckets = s->wfield->buckets_d;
         p.scaling = s->wfield->wakescaling;
@@ -745,6 +478,85 @@ static int step_capture(vfps_step* s, uint32_t steps)
     return VFPS_OK;
 }
 
+// Toeplitz tables and scratch of the direct wake / CSR forms
+static int step_setup_direct(vfps_step* s)
+{
+    const uint32_t n = s->desc.n;
+    const uint32_t M = s->M;
+    const uint32_t nmax = 2 * M;
+    std::vector<float2> z, zc;
+    std::vector<float> rn;
+    if (s->desc.use_wake) {
+        z.resize(nmax);
+        VFPS_HIP(hipMemcpy(z.data(), s->zwake->z, sizeof(float2) * nmax,
+                           hipMemcpyDeviceToHost));
+    }
+    if (s->desc.use_csr) {
+        zc.resize(nmax);
+        rn.resize(nmax);
+        VFPS_HIP(hipMemcpy(zc.data(), s->zcsr->z, sizeof(float2) * nmax,
+                           hipMemcpyDeviceToHost));
+        VFPS_HIP(hipMemcpy(rn.data(), s->csr_renorm, sizeof(float) * nmax,
+                           hipMemcpyDeviceToHost));
+    }
+    std::vector<double> g, h;
+    toeplitz_tables(n, M, z.empty() ? nullptr : z.data(),
+                    zc.empty() ? nullptr : zc.data(),
+                    rn.empty() ? nullptr : rn.data(), s->csr_fdelta,
+                    s->desc.use_wake ? &g : nullptr,
+                    s->desc.use_csr ? &h : nullptr);
+    const size_t nd = 2 * (size_t)n - 1;
+    if (!g.empty()) {
+        VFPS_HIP(hipMalloc(&s->wake_g, sizeof(double) * nd));
+        VFPS_HIP(hipMemcpy(s->wake_g, g.data(), sizeof(double) * nd,
+                           hipMemcpyHostToDevice));
+    }
+    if (!h.empty()) {
+        VFPS_HIP(hipMalloc(&s->csr_h, sizeof(double) * nd));
+        VFPS_HIP(hipMemcpy(s->csr_h, h.data(), sizeof(double) * nd,
+                           hipMemcpyHostToDevice));
+    }
+    VFPS_HIP(hipMalloc(&s->csr_part, sizeof(double) * n));
+    VFPS_HIP(hipMemset(s->csr_part, 0, sizeof(double) * n));
+    VFPS_HIP(hipMalloc(&s->ticket, sizeof(uint32_t)));
+    VFPS_HIP(hipMemset(s->ticket, 0, sizeof(uint32_t)));
+    return VFPS_OK;
+}
+
+namespace vfps_impl {
+
+// change the tile height and workgroup size of a fused step (the results
+// do not depend on them)
+int step_retile(vfps_step* s, uint32_t T, uint32_t threads)
+{
+    VFPS_CHECK_ARG(s && s->fused, "retile: needs a fused step");
+    VFPS_CHECK_ARG(threads % 64 == 0 && threads >= 64 &&
+                       threads <= STEP_MAX_THREADS,
+                   "retile: threads must be a multiple of 64 up to 512");
+    VFPS_CHECK_ARG(T >= 1 && T <= threads && T <= s->desc.n,
+                   "retile: tile rows must be in [1, min(threads, n)]");
+    VFPS_CHECK_ARG(ceil_div(s->desc.n, threads) <= (uint32_t)STEP_MAX_COLS,
+                   "retile: too few threads for the mesh size");
+    const uint32_t oldT = s->T, oldthr = s->threads;
+    s->T = T;
+    s->threads = threads;
+    int rc = step_upload_desc(s);
+    const size_t lds = step_lds_bytes(s->variant, s->desc.n, s->T, s->stride);
+    if (rc == VFPS_OK && lds > s->dev->lds_max) {
+        rc = fail(VFPS_ERR_INVALID, "retile: tile exceeds the LDS of one CU");
+    }
+    if (rc == VFPS_OK) rc = step_set_lds_limit(s->variant, lds);
+    if (rc != VFPS_OK) {
+        s->T = oldT;
+        s->threads = oldthr;
+        step_upload_desc(s);
+        return rc;
+    }
+    return step_drop_graph(s);
+}
+
+}  // namespace vfps_impl
+
 extern "C" {
 
 int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
@@ -755,12 +567,18 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
     VFPS_CHECK_ARG(desc->scheme <= 1, "vfps_step_create: scheme must be 0 or 1");
     VFPS_CHECK_ARG(!desc->use_wake || zwake, "vfps_step_create: wake impedance");
     VFPS_CHECK_ARG(!desc->use_csr || zcsr, "vfps_step_create: CSR impedance");
+    VFPS_CHECK_ARG(desc->wake_method <= 2,
+                   "vfps_step_create: wake_method must be 0, 1 or 2");
+    VFPS_CHECK_ARG(desc->threads == 0 ||
+                       (desc->threads % 64 == 0 && desc->threads <= STEP_MAX_THREADS),
+                   "vfps_step_create: threads must be a multiple of 64 up to 512");
     VFPS_TRY_BEGIN
     auto* s = new vfps_step();
     std::unique_ptr<vfps_step, void (*)(vfps_step*)> guard(s, step_free);
     s->dev = dev;
     s->desc = *desc;
     s->fused = desc->fused != 0;
+    s->direct = s->fused && desc->wake_method != 1;
     s->zwake = zwake;
     s->zcsr = zcsr;
     const uint32_t n = desc->n;
@@ -839,41 +657,73 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
     VFPS_HIP(hipMalloc(&s->filling, sizeof(double) * 2));
     VFPS_HIP(hipMalloc(&s->counter, sizeof(uint32_t)));
     VFPS_HIP(hipMalloc(&s->err, sizeof(uint32_t)));
+    VFPS_HIP(hipMalloc(&s->csr_power, sizeof(float)));
+    VFPS_HIP(hipMemset(s->filling, 0, sizeof(double) * 2));
     VFPS_HIP(hipMemset(s->counter, 0, sizeof(uint32_t)));
     VFPS_HIP(hipMemset(s->err, 0, sizeof(uint32_t)));
+    VFPS_HIP(hipMemset(s->csr_power, 0, sizeof(float)));
     VFPS_HIP(hipMalloc(&s->inst_d, sizeof(InstDesc)));
     VFPS_HIP(hipMalloc(&s->fparams_d, sizeof(FieldParams)));
     VFPS_HIP(hipSetDevice(dev->ordinal));
     VFPS_CHECK_ARG(field_lds_bytes(s->M) <= dev->lds_max,
                    "vfps_step_create: FFT length exceeds the LDS of one CU");
-    // fused kernel variant and tile
+    if (s->direct && (rc = step_setup_direct(s)) != VFPS_OK) return rc;
+    // fused kernel variant, tile and workgroup
     s->variant.scheme = (int)desc->scheme;
     s->variant.it = it;
     s->variant.ipfp = s->fpm ? s->fpm->ip : 0;
-    s->variant.wake = desc->use_wake != 0;
+    s->variant.direct = s->direct;
     s->variant.clamp = desc->rot_clamp != 0;
-    if ((rc = step_upload_desc(s)) != VFPS_OK) return rc;
     if (s->fused) {
-        if (!maps_kernel_for(s->variant)) {
+        if (!step_kernel_for(s->variant)) {
             return fail(VFPS_ERR_UNSUPPORTED,
                         "vfps_step_create: fused step needs linear or cubic "
                         "interpolation (use fused = 0)");
         }
-        uint32_t T = desc->tile_rows ? desc->tile_rows : (n >= 512 ? 4u : 2u);
-        while (T > 1 && fused_lds_bytes(n, T, s->hmax) > dev->lds_max) {
-            T >>= 1;
+        uint32_t thr = desc->threads ? desc->threads : (n >= 512 ? 512u : 256u);
+        if (ceil_div(n, thr) > (uint32_t)STEP_MAX_COLS) {
+            return fail(VFPS_ERR_INVALID, "vfps_step_create: too few threads "
+                                          "for the mesh size");
         }
-        VFPS_CHECK_ARG(fused_lds_bytes(n, T, s->hmax) <= dev->lds_max,
+        s->threads = thr;
+        s->T = desc->tile_rows ? desc->tile_rows : (n >= 1024 ? 2u : 1u);
+        VFPS_CHECK_ARG(s->T <= thr, "vfps_step_create: tile_rows > threads");
+    }
+    if ((rc = step_upload_desc(s)) != VFPS_OK) return rc;
+    if (s->fused) {
+        while (s->T > 1 &&
+               step_lds_bytes(s->variant, n, s->T, s->stride) > dev->lds_max) {
+            s->T >>= 1;
+            if ((rc = step_upload_desc(s)) != VFPS_OK) return rc;
+        }
+        const size_t lds = step_lds_bytes(s->variant, n, s->T, s->stride);
+        VFPS_CHECK_ARG(lds <= dev->lds_max,
                        "vfps_step_create: mesh row too long for the LDS of one "
                        "CU (use fused = 0)");
-        s->T = T;
-        s->threads = 256;
-        if ((rc = fused_set_lds_limit(s->variant,
-                                      fused_lds_bytes(n, T, s->hmax))) != VFPS_OK)
-            return rc;
+        if ((rc = step_set_lds_limit(s->variant, lds)) != VFPS_OK) return rc;
     }
     VFPS_HIP(hipStreamSynchronize(dev->stream));
     *out = guard.release();
+    return VFPS_OK;
+    VFPS_TRY_END
+}
+
+int vfps_toeplitz_kernels(uint32_t nmax, uint32_t n, const vfps_cplx* z,
+                          const vfps_cplx* zc, const float* renorm,
+                          float fdelta, double* g, double* h)
+{
+    VFPS_CHECK_ARG(nmax >= 4 && nmax % 2 == 0 && n >= 1 && n <= nmax / 2,
+                   "vfps_toeplitz_kernels: need an even nmax >= 2 n");
+    VFPS_CHECK_ARG(!g || z, "vfps_toeplitz_kernels: g needs z");
+    VFPS_CHECK_ARG(!h || (zc && renorm), "vfps_toeplitz_kernels: h needs zc "
+                                         "and renorm");
+    VFPS_TRY_BEGIN
+    std::vector<double> gv, hv;
+    toeplitz_tables(n, nmax / 2, reinterpret_cast<const float2*>(z),
+                    reinterpret_cast<const float2*>(zc), renorm, fdelta,
+                    g ? &gv : nullptr, h ? &hv : nullptr);
+    if (g) std::copy(gv.begin(), gv.end(), g);
+    if (h) std::copy(hv.begin(), hv.end(), h);
     return VFPS_OK;
     VFPS_TRY_END
 }
@@ -971,12 +821,14 @@ int vfps_step_get_mesh(vfps_step* s, float* mesh, float* proj, float* wake)
     VFPS_CHECK_ARG(s, "vfps_step_get_mesh: NULL step");
     VFPS_HIP(hipStreamSynchronize(s->dev->stream));
     const size_t n = s->desc.n;
-    const float* cur = (s->fused && (s->parity & 1u)) ? s->t2->data : s->t1->data;
+    const bool odd = s->fused && (s->parity & 1u);
+    const vfps_mesh* cur = odd ? s->t2 : s->t1;
     if (mesh) {
-        VFPS_HIP(hipMemcpy(mesh, cur, sizeof(float) * n * n, hipMemcpyDeviceToHost));
+        VFPS_HIP(hipMemcpy(mesh, cur->data, sizeof(float) * n * n,
+                           hipMemcpyDeviceToHost));
     }
     if (proj) {
-        VFPS_HIP(hipMemcpy(proj, s->t1->proj, sizeof(float) * n,
+        VFPS_HIP(hipMemcpy(proj, cur->proj, sizeof(float) * n,
                            hipMemcpyDeviceToHost));
     }
     if (wake) {
@@ -1003,15 +855,31 @@ int vfps_step_set_mesh(vfps_step* s, const float* mesh)
     return VFPS_OK;
 }
 
+int vfps_step_config(vfps_step* s, uint32_t* out, uint32_t nout)
+{
+    VFPS_CHECK_ARG(s && out && nout >= 6, "vfps_step_config: bad argument");
+    out[0] = s->fused ? 1u : 0u;
+    out[1] = s->direct ? 2u : 1u;
+    out[2] = s->T;
+    out[3] = s->threads;
+    out[4] = s->stride;
+    out[5] = s->fused ? (uint32_t)step_lds_bytes(s->variant, s->desc.n, s->T,
+                                                 s->stride)
+                      : 0u;
+    return VFPS_OK;
+}
+
 int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
 {
     VFPS_CHECK_ARG(s && ns_out && nout >= 16, "vfps_step_phases: bad argument");
     VFPS_CHECK_ARG(s->fused, "vfps_step_phases: needs the fused step");
     hipStream_t st = s->dev->stream;
     VFPS_HIP(hipStreamSynchronize(st));
+    const uint32_t nblk = s->tiles();
+    const size_t nstamp = 16 + 2 * (size_t)nblk;
     unsigned long long* buf = nullptr;
-    VFPS_HIP(hipMalloc(&buf, sizeof(unsigned long long) * 16));
-    VFPS_HIP(hipMemset(buf, 0, sizeof(unsigned long long) * 16));
+    VFPS_HIP(hipMalloc(&buf, sizeof(unsigned long long) * nstamp));
+    VFPS_HIP(hipMemset(buf, 0, sizeof(unsigned long long) * nstamp));
     // point both descriptors at the stamp buffer for one step
     unsigned long long* mstamp = buf;
     unsigned long long* fstamp = buf + 8;
@@ -1022,8 +890,9 @@ int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
     int rc = step_launch_one(s, s->parity);
     s->parity ^= 1u;
     hipError_t e = hipStreamSynchronize(st);
-    unsigned long long h[16] = {0};
-    hipMemcpy(h, buf, sizeof(h), hipMemcpyDeviceToHost);
+    std::vector<unsigned long long> h(nstamp, 0);
+    hipMemcpy(h.data(), buf, sizeof(unsigned long long) * nstamp,
+              hipMemcpyDeviceToHost);
     unsigned long long* nul = nullptr;
     hipMemcpy(reinterpret_cast<char*>(s->inst_d) + offsetof(InstDesc, tstamp),
               &nul, sizeof(nul), hipMemcpyHostToDevice);
@@ -1036,16 +905,41 @@ int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
     VFPS_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
                                    s->dev->ordinal));
     const double ns_per_tick = khz > 0 ? 1e6 / khz : 10.0;
-    // [0..6]: maps kernel phase ends relative to its start;
-    // [8..13]: field kernel phase ends relative to its start; 15: tick ns
-    for (uint32_t i = 0; i < 16; i++) ns_out[i] = 0;
-    for (int i = 1; i < 7; i++) {
+    // [1..7]: step kernel phase ends (tile 0) relative to its start;
+    // [9..13]: field kernel phase ends relative to its start; 15: tick ns;
+    // [16..21] (nout >= 24): step workgroups: first entry after the field
+    // kernel's last stamp, spread of entries, shortest, median and longest
+    // workgroup, last exit after first entry
+    for (uint32_t i = 0; i < nout; i++) ns_out[i] = 0;
+    for (int i = 1; i < 8; i++) {
         if (h[i] && h[0]) ns_out[i] = (double)(h[i] - h[0]) * ns_per_tick;
     }
     for (int i = 9; i < 14; i++) {
         if (h[i] && h[8]) ns_out[i] = (double)(h[i] - h[8]) * ns_per_tick;
     }
     ns_out[15] = ns_per_tick;
+    if (nout >= 24) {
+        unsigned long long e0 = ~0ull, e1 = 0, x1 = 0;
+        std::vector<double> dur;
+        for (uint32_t b = 0; b < nblk; b++) {
+            const unsigned long long a = h[16 + 2 * b], z = h[17 + 2 * b];
+            if (!a || !z) continue;
+            e0 = std::min(e0, a);
+            e1 = std::max(e1, a);
+            x1 = std::max(x1, z);
+            dur.push_back((double)(z - a) * ns_per_tick);
+        }
+        if (!dur.empty()) {
+            std::sort(dur.begin(), dur.end());
+            const unsigned long long fend = h[13];
+            ns_out[16] = fend ? ((double)e0 - (double)fend) * ns_per_tick : 0;
+            ns_out[17] = (double)(e1 - e0) * ns_per_tick;
+            ns_out[18] = dur.front();
+            ns_out[19] = dur[dur.size() / 2];
+            ns_out[20] = dur.back();
+            ns_out[21] = (double)(x1 - e0) * ns_per_tick;
+        }
+    }
     return VFPS_OK;
 }
 
@@ -1053,38 +947,37 @@ int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
                      float* ms_maps, float* ms_step)
 {
     VFPS_CHECK_ARG(s && nsteps > 0, "vfps_step_timing: bad argument");
+    VFPS_CHECK_ARG(s->fused, "vfps_step_timing: per-kernel timing needs the "
+                             "fused step");
     hipStream_t st = s->dev->stream;
     hipEvent_t e0, e1, e2;
     VFPS_HIP(hipEventCreate(&e0));
     VFPS_HIP(hipEventCreate(&e1));
     VFPS_HIP(hipEventCreate(&e2));
-    const bool wake = s->desc.use_wake != 0;
-    const bool csr = s->desc.use_csr != 0;
     double tw = 0, tm = 0;
     int rc = VFPS_OK;
     for (uint32_t k = 0; k < nsteps && rc == VFPS_OK; k++) {
-        const uint32_t par = s->fused ? s->parity : 0;
+        const uint32_t par = s->parity;
         hipEventRecord(e0, st);
-        rc = launch_fields(wake, csr, s->fparams_d, 1, s->M, st);
+        if (!s->direct) {
+            rc = launch_fields(s->desc.use_wake != 0, s->desc.use_csr != 0,
+                               s->fparams_d, 1, s->M, par, st);
+        }
         hipEventRecord(e1, st);
         if (rc == VFPS_OK) {
-            if (s->fused) {
-                rc = launch_fused_maps(s->variant, s->inst_d, 1, s->desc.n,
-                                       s->T, s->threads, s->hmax, par, st);
-            } else {
-                rc = VFPS_ERR_UNSUPPORTED;
-                set_error("vfps_step_timing: per-kernel timing needs the "
-                          "fused step");
-            }
+            rc = launch_step_kernel(s->variant, s->inst_d, nullptr, s->tiles(),
+                                    step_lds_bytes(s->variant, s->desc.n, s->T,
+                                                   s->stride),
+                                    s->threads, par, st);
         }
         hipEventRecord(e2, st);
         hipEventSynchronize(e2);
         float a = 0, b = 0;
         hipEventElapsedTime(&a, e0, e1);
         hipEventElapsedTime(&b, e1, e2);
-        tw += a;
+        if (!s->direct) tw += a;
         tm += b;
-        if (s->fused) s->parity ^= 1u;
+        s->parity ^= 1u;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);

@@ -9,47 +9,70 @@
 
 namespace vfps_impl {
 
-// One instance (one mesh) as the fused map kernel sees it.
+// One instance (one mesh) as the step kernel sees it.
 struct InstDesc {
     uint32_t n;
     uint32_t hlo, hhi;            // halo rows needed by the drift/rotation
-    float* mesh0;                 // ping-pong state buffers
-    float* mesh1;
-    float* proj;                  // [n] x projection (written every step)
+    uint32_t tiles;               // row tiles of this instance
+    uint32_t T;                   // output rows per tile
+    uint32_t stride;              // LDS buffer rows (>= T + hlo + hhi)
+    float* mesh[2];               // ping-pong state buffers
+    float* proj[2];               // x projection of mesh[k]
     const float* ws;              // [n] Simpson weights
-    const float* wake_off;        // [n] wake offsets (nullptr: no wake map)
+    const float* wake_off;        // FFT wake: [n] offsets of the field kernel
+    const double* wake_g;         // direct wake: g[d + n - 1], |d| < n
+    float* wake_out;              // direct wake: [n] offsets of the step
+    float wake_scale;             // wake scaling divided by nmax
+    int32_t use_wake, use_csr;
+    const double* csr_h;          // direct CSR: h[d + n - 1], |d| < n
+    double* csr_part;             // [tiles] CSR shares of the tiles
+    uint32_t* ticket;             // tiles finished in this step
+    float* csr_power;             // [1] CSR power of the last step
+    double* filling;              // [2] FP64 charge integral (+ total)
     const float* rf_off;          // [n] RF kick offsets (scheme 0)
     const float* drift_off;       // [n] drift offsets (scheme 0)
     const float* fp_w;            // [ip][n] Fokker-Planck weights
     const uint32_t* fp_i;         // [ip][n] Fokker-Planck indices
     RotGeom rot;                  // scheme 1
     float* prof_hist;             // optional per-step profile history
+    float* csr_hist;              // optional per-step CSR power history
     uint32_t hist_cap;
-    const uint32_t* counter;      // steps counted by the field kernel
+    uint32_t* counter;            // completed steps
     uint32_t* err;                // set if a tap falls outside the halo
     uint32_t seq_proj;            // 1: x projection in the reference's order
-    unsigned long long* tstamp;   // optional phase timestamps of workgroup 0
+    unsigned long long* tstamp;   // optional timestamps (phases of tile 0,
+                                  // entry/exit of every workgroup)
 };
 
 // kernel variant key: scheme (0 kick-drift, 1 rotation), interpolation,
-// FP taps (0 = identity), wake on/off, clamp
-struct MapsVariant {
+// FP taps (0 = identity), direct wake (else the field kernel's offsets),
+// clamp
+struct StepVariant {
     int scheme, it, ipfp;
-    bool wake, clamp;
+    bool direct, clamp;
 };
 
-// Launch the fused maps kernel over `ninst` instances of the same n with
-// `tiles` row tiles each (tile height T rows) and `threads` per block.
-int launch_fused_maps(const MapsVariant& v, const InstDesc* descs_d,
-                      uint32_t ninst, uint32_t n, uint32_t T,
-                      uint32_t threads, uint32_t hmax, uint32_t parity,
-                      hipStream_t st);
-size_t fused_lds_bytes(uint32_t n, uint32_t T, uint32_t hmax);
-int fused_set_lds_limit(const MapsVariant& v, size_t bytes);
+const void* step_kernel_for(const StepVariant& v);
+size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
+                      uint32_t stride);
+int step_set_lds_limit(const StepVariant& v, size_t bytes);
+// Launch the step kernel: `nblocks` workgroups of `threads` with
+// `lds_bytes` of dynamic LDS (at least step_lds_bytes of every instance);
+// blockmap_d (optional) gives (instance << 16 | tile) per logical
+// workgroup (STEP_NO_TILE: padding), else instance 0, tile = workgroup.
+int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
+                       const uint32_t* blockmap_d, uint32_t nblocks,
+                       size_t lds_bytes, uint32_t threads, uint32_t parity,
+                       hipStream_t st);
+// FP64 Toeplitz kernels of the direct wake (g) and CSR power (h) forms
+void toeplitz_tables(uint32_t n, uint32_t M, const float2* Z,
+                     const float2* Zc, const float* renorm, float fdelta,
+                     std::vector<double>* g, std::vector<double>* h);
 
 // Field kernel over `ninst` instances (one workgroup each).
 int launch_fields(bool wake, bool csr, const FieldParams* params_d,
-                  uint32_t ninst, uint32_t Mmax, hipStream_t st);
+                  uint32_t ninst, uint32_t Mmax, uint32_t parity,
+                  hipStream_t st);
 
 // exact halo of a kick along x (drift) with the given offsets
 void kick_halo(const std::vector<float>& off, uint32_t n, int it,
@@ -58,4 +81,47 @@ void kick_halo(const std::vector<float>& off, uint32_t n, int it,
 void rotation_halo(const RotGeom& g, uint32_t n, int it, uint32_t* hlo,
                    uint32_t* hhi);
 
+}  // namespace vfps_impl
+
+// A whole step on one mesh (the object behind vfps_step_*).
+struct vfps_step {
+    vfps_dev* dev = nullptr;
+    vfps_step_desc desc{};
+    vfps_mesh *t1 = nullptr, *t2 = nullptr, *t3 = nullptr;
+    vfps_map *wm = nullptr, *rfm = nullptr, *drm = nullptr, *rot = nullptr,
+             *fpm = nullptr;
+    vfps_field* wfield = nullptr;
+    vfps_impedance* zwake = nullptr;
+    vfps_impedance* zcsr = nullptr;
+    float* csr_renorm = nullptr;
+    double2* csr_tw = nullptr;
+    float csr_fdelta = 0;
+    double* filling = nullptr;  // [2]
+    uint32_t* counter = nullptr;
+    uint32_t* err = nullptr;
+    float* csr_hist = nullptr;
+    float* prof_hist = nullptr;
+    uint32_t hist_cap = 0;
+    // direct wake / CSR forms
+    double* wake_g = nullptr;   // [2n-1]
+    double* csr_h = nullptr;    // [2n-1]
+    double* csr_part = nullptr; // [tiles]
+    uint32_t* ticket = nullptr;
+    float* csr_power = nullptr;
+    vfps_impl::InstDesc* inst_d = nullptr;
+    vfps_impl::FieldParams* fparams_d = nullptr;
+    vfps_impl::StepVariant variant{};
+    uint32_t T = 1, threads = 256, stride = 1, hmax = 0, M = 0;
+    bool fused = true;
+    bool direct = true;
+    uint32_t parity = 0;  // fused path: buffer holding the state (0: t1)
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    uint32_t graph_steps = 0;
+
+    uint32_t tiles() const { return vfps_impl::ceil_div(desc.n, T); }
+};
+
+namespace vfps_impl {
+int step_retile(vfps_step* s, uint32_t T, uint32_t threads);
 }  // namespace vfps_impl

@@ -1,0 +1,665 @@
+// The fused time step of src/main.cpp:948-1053: one kernel per step.
+//
+// A workgroup owns a tile of T output rows of one mesh (one instance) and
+// holds, in LDS, the input rows the tile depends on (the tile plus the halo
+// rows the drift or rotation reads).  It runs, in order:
+//   A  every global read: the input rows, per-row kick offsets, per-column
+//      drift/Fokker-Planck/Simpson tables and, for the direct wake, the
+//      previous x projection and the window of the wake kernel the tile's
+//      rows need;
+//   B  (direct wake) the wake offsets of its rows as the Toeplitz product of
+//      the previous profile with the kernel g (FP64), and its share of the
+//      CSR power as the quadratic form p^T H p;
+//   C  wake kick along y (WakePotentialMap::apply);
+//   D  RF kick along y (scheme 0, RFKickMap::apply);
+//   E  drift along x (scheme 0, DriftMap::apply) or rotation (scheme 1);
+//   F  Fokker-Planck along y, the store of the output rows, and the
+//      products mesh*weight of the x projection;
+//   G  the x projection of its rows (PhaseSpace::updateXProjection), summed
+//      in std::inner_product order;
+//   H  (direct wake) the workgroup that finishes last sums the CSR shares in
+//      a fixed order, computes the FP64 charge integral and counts the step.
+// The mesh makes one trip HBM -> LDS -> HBM per step.  The state buffers
+// and the x projections ping-pong (`parity`), so a step reads only what
+// the previous launch wrote.
+//
+// Every map keeps the reference's float arithmetic and operation order, so
+// with the same wake offsets the output equals the per-map kernels bit for
+// bit.  The direct wake differs from the FFT of the reference only by
+// rounding: g[d] = Re Z_0 + 2 sum_{k=1}^{M-1} Re(Z_k e^{2 pi i k d/N}) is
+// the exact impulse response of "zero-pad, R2C, times Z, C2R" of
+// src/PS/ElectricField.cpp:215-241 (Nyquist bin zero, Im W_0 ignored by
+// the C2R), and the product is accumulated in FP64 and rounded to float
+// where FFTW's output is float.
+#pragma once
+
+#include "common.hpp"
+#include "maps.hpp"
+#include "reduce.hpp"
+#include "step.hpp"
+
+namespace vfps_impl {
+
+// LDS layout of the step kernel (offsets in 4-byte words; the FP64 arrays
+// come first so they stay 8-byte aligned)
+struct StepLds {
+    uint32_t gwin, hwin;          // FP64 kernel windows (word offsets)
+    uint32_t pprev;               // previous x projection
+    uint32_t bufA, bufB;          // row buffers, `stride` rows each
+    uint32_t tile_out;            // [T][n + 4] projection products
+    uint32_t offw, offr;          // per-row kick offsets
+    uint32_t tapw, tapr;          // per-row taps: IT shifts, then IT weights
+    uint32_t csr_rows;            // FP64 CSR share of each output row
+    uint32_t words;
+};
+
+__host__ __device__ inline uint32_t step_round4(uint32_t v) { return (v + 3u) & ~3u; }
+
+__host__ __device__ inline StepLds step_lds(uint32_t n, uint32_t T, uint32_t stride,
+                                            bool direct, uint32_t it)
+{
+    StepLds L{};
+    uint32_t w = 0;
+    if (direct) {
+        L.gwin = w;
+        w += 2 * (n + stride);
+        L.hwin = w;
+        w += 2 * (n + T);
+        L.csr_rows = w;
+        w += 2 * T;
+        w = step_round4(w);
+        L.pprev = w;
+        w += step_round4(n);
+    }
+    L.bufA = w;
+    w += step_round4(stride * n);
+    L.bufB = w;
+    w += step_round4(stride * n);
+    L.tile_out = w;
+    w += T * (n + 4);
+    L.offw = w;
+    w += step_round4(stride);
+    L.offr = w;
+    w += step_round4(stride);
+    L.tapw = w;
+    w += step_round4(2 * it * stride);
+    L.tapr = w;
+    w += step_round4(2 * it * stride);
+    L.words = w;
+    return L;
+}
+
+// global-memory views of pointers read from a descriptor (without them the
+// compiler sees generic pointers and emits flat accesses)
+typedef float vec4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) vec4f gfloat4;
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) uint32_t guint;
+
+// most mesh columns one thread handles
+constexpr int STEP_MAX_COLS = 4;
+// float4 loads one thread keeps in flight while filling the tile
+constexpr int STEP_LOAD_BATCH = 8;
+// largest workgroup
+constexpr uint32_t STEP_MAX_THREADS = 512;
+// block-map entry of a padding workgroup
+constexpr uint32_t STEP_NO_TILE = 0xFFFFFFFFu;
+
+#define VFPS_STAMP(k)                                                         \
+    do {                                                                      \
+        if (d.tstamp && tile == 0 && inst == 0) {                             \
+            __syncthreads();                                                  \
+            if (threadIdx.x == 0) d.tstamp[k] = wall_clock64();               \
+        }                                                                     \
+    } while (0)
+
+// taps of one kick row, stored as IT shifts (index - n/2) then IT weights
+template <int IT>
+__device__ __forceinline__ void store_row_taps(float off, uint32_t n, float* taps)
+{
+    uint32_t idx[IT];
+    float w[IT];
+    kick_taps<IT>(off, n, idx, w);
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+        taps[j] = __int_as_float((int32_t)idx[j] - (int32_t)(n / 2));
+        taps[IT + j] = w[j];
+    }
+}
+
+// KickMap::apply along y (src/SM/KickMap.cpp:244-267) on local rows
+// [r0, r1) of a row buffer, every (row, y) pair spread over the workgroup
+template <int IT>
+__device__ __forceinline__ void kick_rows(const float* __restrict__ src,
+                                          float* __restrict__ dst, uint32_t n,
+                                          int r0, int r1,
+                                          const float* __restrict__ taps)
+{
+    if (r1 <= r0) {
+        return;
+    }
+    const uint32_t bs = blockDim.x;
+    const uint32_t drow = bs / n, dy = bs % n;
+    uint32_t row = (uint32_t)r0 + threadIdx.x / n;
+    uint32_t y = threadIdx.x % n;
+#pragma unroll 2
+    for (; row < (uint32_t)r1;) {
+        const float* tp = taps + row * (2 * IT);
+        const float* s = src + (size_t)row * n;
+        float v = 0.0f;
+#pragma unroll
+        for (int j = 0; j < IT; j++) {
+            const uint32_t ys = (uint32_t)((int32_t)y + __float_as_int(tp[j]));
+            if (ys < n) {
+                v = v + s[ys] * tp[IT + j];
+            }
+        }
+        dst[(size_t)row * n + y] = v;
+        y += dy;
+        row += drow;
+        if (y >= n) {
+            y -= n;
+            row++;
+        }
+    }
+}
+
+// Toeplitz dot product of the previous profile with a kernel window:
+// sum_x' win[base + n - 1 - x'] p[x'] in FP64, over one wavefront
+__device__ __forceinline__ double toeplitz_dot(const double* __restrict__ win,
+                                               uint32_t base,
+                                               const float* __restrict__ p,
+                                               uint32_t n, uint32_t lane)
+{
+    double acc = 0.0;
+    for (uint32_t x = lane; x < n; x += 64) {
+        acc += win[base + n - 1 - x] * (double)p[x];
+    }
+    return wave_sum(acc);
+}
+
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP>
+__global__ void __launch_bounds__(STEP_MAX_THREADS)
+step_kernel(const InstDesc* __restrict__ descs,
+            const uint32_t* __restrict__ blockmap, uint32_t parity)
+{
+    extern __shared__ __align__(16) float lds[];
+    __shared__ double red[16];
+    __shared__ float s00_sh;
+    __shared__ uint32_t slot_sh;
+    __shared__ uint32_t last_sh;
+    const unsigned long long t_entry = wall_clock64();
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
+    // so give each XCD a contiguous run of tiles (neighbouring tiles share
+    // halo rows, which then hit in that XCD's L2).
+    const uint32_t nblocks = gridDim.x;
+    uint32_t logical = blockIdx.x;
+    if ((nblocks & 7u) == 0u) {
+        logical = (blockIdx.x & 7u) * (nblocks >> 3) + (blockIdx.x >> 3);
+    }
+    uint32_t inst = 0, tile = logical;
+    if (blockmap) {
+        const uint32_t m = blockmap[logical];
+        if (m == STEP_NO_TILE) {
+            return;  // padding of the grid to a multiple of the XCD count
+        }
+        inst = m >> 16;
+        tile = m & 0xFFFFu;
+    }
+    const InstDesc& d = descs[inst];
+    const uint32_t n = d.n;
+    const uint32_t T = d.T;
+    const uint32_t stride = d.stride;
+    const int x0 = (int)(tile * T);
+    if (x0 >= (int)n) {
+        return;  // not launched: grids hold exactly the tiles
+    }
+    VFPS_STAMP(0);
+    const int rows_out = min((int)T, (int)n - x0);
+    const int hlo = (int)d.hlo;
+    const int rbase = x0 - hlo;
+    const int R = rows_out + hlo + (int)d.hhi;
+    // rows of the buffer that exist in the mesh
+    const int lr0 = max(0, -rbase);
+    const int lr1 = min(R, (int)n - rbase);
+    const StepLds L = step_lds(n, T, stride, DIRECT, IT);
+    float* bufA = lds + L.bufA;
+    float* bufB = lds + L.bufB;
+    float* tile_out = lds + L.tile_out;
+    float* offw = lds + L.offw;
+    float* offr = lds + L.offr;
+    float* tapw = lds + L.tapw;
+    float* tapr = lds + L.tapr;
+    double* gwin = reinterpret_cast<double*>(lds + L.gwin);
+    double* hwin = reinterpret_cast<double*>(lds + L.hwin);
+    double* csr_rows = reinterpret_cast<double*>(lds + L.csr_rows);
+    float* pprev = lds + L.pprev;
+    const gfloat* in = (const gfloat*)d.mesh[parity];
+    gfloat* out = (gfloat*)d.mesh[parity ^ 1u];
+    const gfloat* proj_in = (const gfloat*)d.proj[parity];
+    gfloat* proj_out = (gfloat*)d.proj[parity ^ 1u];
+    gfloat* prof_hist = (gfloat*)d.prof_hist;
+    const gfloat* ws_g = (const gfloat*)d.ws;
+    const gfloat* rf_off = (const gfloat*)d.rf_off;
+    const gfloat* drift_off = (const gfloat*)d.drift_off;
+    const gfloat* wake_off = (const gfloat*)d.wake_off;
+    const gfloat* fp_w = (const gfloat*)d.fp_w;
+    const guint* fp_i = (const guint*)d.fp_i;
+    const gdouble* wake_g = (const gdouble*)d.wake_g;
+    const gdouble* csr_h = (const gdouble*)d.csr_h;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t lane = tid & 63u, wave = tid >> 6, nw = bs >> 6;
+    const int32_t half = (int32_t)(n / 2);
+    const bool wake = d.use_wake != 0;
+    const bool csr = DIRECT && d.use_csr != 0;
+
+    // ---- A: every global read -------------------------------------------
+    uint32_t cnt = 0;
+    if (d.counter && tid == 0) {
+        cnt = *d.counter;
+    }
+    //  - previous profile and the kernel windows (direct wake)
+    //    wake rows [rbase+lr0, rbase+lr1) need g[r - x' + n - 1]:
+    //    gwin[k] = g[rbase + lr0 + k], k < (lr1 - lr0) + n - 1;
+    //    CSR rows [x0, x0+rows_out) need hwin[k] = h[x0 + k]
+    if (DIRECT) {
+        for (uint32_t x = tid; x < n; x += bs) {
+            pprev[x] = proj_in[x];
+        }
+        if (wake) {
+            const uint32_t g0 = (uint32_t)(rbase + lr0);
+            const uint32_t gl = (uint32_t)(lr1 - lr0) + n - 1;
+            for (uint32_t k = tid; k < gl; k += bs) {
+                gwin[k] = wake_g[g0 + k];
+            }
+        }
+        if (csr) {
+            const uint32_t hl = (uint32_t)rows_out + n - 1;
+            for (uint32_t k = tid; k < hl; k += bs) {
+                hwin[k] = csr_h[(uint32_t)x0 + k];
+            }
+        }
+    }
+    //  - per-row kick offsets; the RF taps (and FFT-wake taps) right away
+    if ((int)tid >= lr0 && (int)tid < lr1) {
+        const int r = rbase + (int)tid;
+        if (SCHEME == 0) {
+            offr[tid] = rf_off[r];
+        }
+        if (!DIRECT && wake) {
+            offw[tid] = wake_off[r];
+        }
+    }
+    //  - per-column tables in registers
+    const int ncols = (int)((n + bs - 1) / bs);
+    float col_drift[STEP_MAX_COLS], col_ws[STEP_MAX_COLS];
+    uint32_t col_fi[STEP_MAX_COLS][IPFP > 0 ? IPFP : 1];
+    float col_fw[STEP_MAX_COLS][IPFP > 0 ? IPFP : 1];
+#pragma unroll
+    for (int c = 0; c < STEP_MAX_COLS; c++) {
+        const uint32_t y = tid + (uint32_t)c * bs;
+        if (c < ncols && y < n) {
+            if (SCHEME == 0) col_drift[c] = drift_off[y];
+            col_ws[c] = ws_g[y];
+#pragma unroll
+            for (int k = 0; k < IPFP; k++) {
+                col_fi[c][k] = fp_i[(size_t)k * n + y];
+                col_fw[c][k] = fp_w[(size_t)k * n + y];
+            }
+        }
+    }
+    //  - input rows [rbase+lr0, rbase+lr1)
+    {
+        const size_t first = (size_t)(rbase + lr0) * n;
+        const uint32_t total = (uint32_t)(lr1 - lr0) * n;
+        float* dst = bufA + (size_t)lr0 * n;
+        if ((n & 3u) == 0u) {
+            const gfloat4* in4 = (const gfloat4*)(in + first);
+            vec4f* a4 = reinterpret_cast<vec4f*>(dst);
+            const uint32_t total4 = total >> 2;
+            for (uint32_t base = 0; base < total4; base += bs * STEP_LOAD_BATCH) {
+                // unconditional loads (clamped index) keep v in registers
+                vec4f v[STEP_LOAD_BATCH];
+#pragma unroll
+                for (int b = 0; b < STEP_LOAD_BATCH; b++) {
+                    const uint32_t i = base + tid + (uint32_t)b * bs;
+                    v[b] = in4[i < total4 ? i : total4 - 1u];
+                }
+#pragma unroll
+                for (int b = 0; b < STEP_LOAD_BATCH; b++) {
+                    const uint32_t i = base + tid + (uint32_t)b * bs;
+                    if (i < total4) a4[i] = v[b];
+                }
+            }
+        } else {
+            for (uint32_t i = tid; i < total; i += bs) {
+                dst[i] = in[first + i];
+            }
+        }
+    }
+    if ((int)tid >= lr0 && (int)tid < lr1) {
+        if (SCHEME == 0) {
+            store_row_taps<IT>(offr[tid], n, tapr + tid * (2 * IT));
+        }
+        if (!DIRECT && wake) {
+            store_row_taps<IT>(offw[tid], n, tapw + tid * (2 * IT));
+        }
+    }
+    __syncthreads();
+    VFPS_STAMP(1);
+
+    // ---- B: direct wake offsets and the CSR share --------------------------
+    // row 0 of the wake-kicked input (scheme 1 reads it for taps out of
+    // range); tiles that do not hold row 0 compute it themselves
+    const bool need_s00 = SCHEME == 1 && rbase > 0;
+    double csr_q = 0.0;
+    if (DIRECT && (wake || csr)) {
+        const int nwk = wake ? lr1 - lr0 : 0;
+        const int ncsr = csr ? rows_out : 0;
+        const int njobs = nwk + ncsr + ((need_s00 && wake) ? 1 : 0);
+        for (int job = (int)wave; job < njobs; job += (int)nw) {
+            if (job < nwk) {
+                const double a = toeplitz_dot(gwin, (uint32_t)job, pprev, n, lane);
+                if (lane == 0) {
+                    const int row = lr0 + job;
+                    const float off = d.wake_scale * (float)a;
+                    offw[row] = off;
+                    store_row_taps<IT>(off, n, tapw + row * (2 * IT));
+                    if (row >= hlo && row < hlo + rows_out && d.wake_out) {
+                        d.wake_out[rbase + row] = off;
+                    }
+                }
+            } else if (job < nwk + ncsr) {
+                const int o = job - nwk;
+                const double u = toeplitz_dot(hwin, (uint32_t)o, pprev, n, lane);
+                if (lane == 0) {
+                    csr_rows[o] = (double)pprev[x0 + o] * u;
+                }
+            } else {
+                // wake offset of row 0 from the kernel in global memory
+                double acc = 0.0;
+                for (uint32_t x = lane; x < n; x += 64) {
+                    acc += wake_g[n - 1 - x] * (double)pprev[x];
+                }
+                acc = wave_sum(acc);
+                if (lane == 0) {
+                    const float off = d.wake_scale * (float)acc;
+                    uint32_t idx[IT];
+                    float w[IT];
+                    kick_taps<IT>(off, n, idx, w);
+                    float v = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < IT; j++) {
+                        const uint32_t ys = (uint32_t)((int32_t)idx[j] - half);
+                        if (ys < n) {
+                            v = v + in[ys] * w[j];
+                        }
+                    }
+                    s00_sh = v;
+                }
+            }
+        }
+        __syncthreads();
+        if (csr && tid == 0) {
+            for (int o = 0; o < rows_out; o++) {
+                csr_q += csr_rows[o];
+            }
+        }
+    }
+    VFPS_STAMP(2);
+
+    float* cur = bufA;
+    float* oth = bufB;
+    // ---- C: wake kick along y ----------------------------------------------
+    if (wake) {
+        kick_rows<IT>(cur, oth, n, lr0, lr1, tapw);
+        __syncthreads();
+        float* t = cur;
+        cur = oth;
+        oth = t;
+    }
+    VFPS_STAMP(3);
+    if constexpr (SCHEME == 0) {
+        // ---- D: RF kick along y --------------------------------------------
+        kick_rows<IT>(cur, oth, n, lr0, lr1, tapr);
+        __syncthreads();
+        float* t = cur;
+        cur = oth;
+        oth = t;
+        VFPS_STAMP(4);
+        // ---- E: drift along x, one column per thread ----------------------
+#pragma unroll
+        for (int c = 0; c < STEP_MAX_COLS; c++) {
+            const uint32_t y = tid + (uint32_t)c * bs;
+            if (c >= ncols || y >= n) {
+                continue;
+            }
+            uint32_t idx[IT];
+            float w[IT];
+            kick_taps<IT>(col_drift[c], n, idx, w);
+            for (int orow = 0; orow < rows_out; orow++) {
+                const uint32_t x = (uint32_t)(x0 + orow);
+                float v = 0.0f;
+#pragma unroll
+                for (int j = 0; j < IT; j++) {
+                    const uint32_t xs = (uint32_t)((int32_t)(x + idx[j]) - half);
+                    if (xs < n) {
+                        int lr = (int)xs - rbase;
+                        if (lr < lr0 || lr >= lr1) {
+                            atomicOr(d.err, 1u);
+                            lr = lr < lr0 ? lr0 : lr1 - 1;
+                        }
+                        v = v + cur[(size_t)lr * n + y] * w[j];
+                    }
+                }
+                oth[(size_t)orow * n + y] = v;
+            }
+        }
+    } else {
+        // ---- E: rotation (RotationMap::apply).  Taps out of range read
+        // sample 0 of the (wake-kicked) input, as the reference does.
+        float s00;
+        if (rbase <= 0) {
+            s00 = cur[(size_t)(-rbase) * n];
+        } else if (wake) {
+            if (DIRECT) {
+                s00 = s00_sh;
+            } else {
+                uint32_t idx[IT];
+                float w[IT];
+                kick_taps<IT>(wake_off[0], n, idx, w);
+                float v = 0.0f;
+#pragma unroll
+                for (int j = 0; j < IT; j++) {
+                    const uint32_t ys = (uint32_t)((int32_t)idx[j] - half);
+                    if (ys < n) {
+                        v = v + in[ys] * w[j];
+                    }
+                }
+                s00 = v;
+            }
+        } else {
+            s00 = in[0];
+        }
+        VFPS_STAMP(4);
+        const RotGeom g = d.rot;
+        const float* src = cur;
+        const uint32_t drow = bs / n, dy = bs % n;
+        uint32_t orow = tid / n, y0 = tid % n;
+        while (orow < (uint32_t)rows_out) {
+            const float v = rotate_cell_src<IT, CLAMP>(
+                n, g, (uint32_t)x0 + orow, y0,
+                [&](bool ok, uint32_t i0, uint32_t j0) -> float {
+                    if (!ok) {
+                        return s00;
+                    }
+                    int lr = (int)i0 - rbase;
+                    if (lr < lr0 || lr >= lr1) {
+                        atomicOr(d.err, 2u);
+                        lr = lr < lr0 ? lr0 : lr1 - 1;
+                    }
+                    return src[(size_t)lr * n + j0];
+                });
+            oth[(size_t)orow * n + y0] = v;
+            y0 += dy;
+            orow += drow;
+            if (y0 >= n) {
+                y0 -= n;
+                orow++;
+            }
+        }
+    }
+    __syncthreads();
+    VFPS_STAMP(5);
+    // ---- F: Fokker-Planck along y, store, projection products ----------------
+    const uint32_t tstride = n + 4;
+#pragma unroll
+    for (int c = 0; c < STEP_MAX_COLS; c++) {
+        const uint32_t y = tid + (uint32_t)c * bs;
+        if (c >= ncols || y >= n) {
+            continue;
+        }
+        for (int orow = 0; orow < rows_out; orow++) {
+            const float* src = oth + (size_t)orow * n;
+            float v;
+            if constexpr (IPFP == 0) {
+                v = src[y];
+            } else {
+                v = 0.0f;
+#pragma unroll
+                for (int k = 0; k < IPFP; k++) {
+                    v = v + src[col_fi[c][k]] * col_fw[c][k];
+                }
+            }
+            out[(size_t)(x0 + orow) * n + y] = v;
+            tile_out[(size_t)orow * tstride + y] = v * col_ws[c];
+        }
+    }
+    if (tid == 0) {
+        // history slot of this step (the counter before the step in the
+        // direct form; the field kernel has already counted it otherwise)
+        uint32_t slot = 0xFFFFFFFFu;
+        if (d.prof_hist && d.counter) {
+            const uint32_t c = DIRECT ? cnt : cnt - 1u;
+            if ((DIRECT || cnt >= 1) && c < d.hist_cap) {
+                slot = c;
+            }
+        }
+        slot_sh = slot;
+    }
+    __syncthreads();
+    const uint32_t slot = slot_sh;
+    VFPS_STAMP(6);
+    // ---- G: x projection ------------------------------------------------------
+    if (d.seq_proj) {
+        // the reference's std::inner_product order: y = 0..n-1, one thread
+        // per row, loads batched ahead of the dependent adds
+        if ((int)tid < rows_out) {
+            const float* r = tile_out + (size_t)tid * tstride;
+            float acc = 0.0f;
+            uint32_t y = 0;
+            if ((n & 3u) == 0u) {
+                const float4* r4 = reinterpret_cast<const float4*>(r);
+                for (; y + 16 <= n; y += 16) {
+                    float4 q[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        q[k] = r4[(y >> 2) + k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        acc = acc + q[k].x;
+                        acc = acc + q[k].y;
+                        acc = acc + q[k].z;
+                        acc = acc + q[k].w;
+                    }
+                }
+            }
+            for (; y < n; y++) {
+                acc = acc + r[y];
+            }
+            const uint32_t x = (uint32_t)x0 + tid;
+            proj_out[x] = acc;
+            if (slot != 0xFFFFFFFFu) {
+                prof_hist[(size_t)slot * n + x] = acc;
+            }
+        }
+    } else {
+        for (int orow = (int)wave; orow < rows_out; orow += (int)nw) {
+            const float* r = tile_out + (size_t)orow * tstride;
+            float acc = 0.0f;
+            for (uint32_t y = lane; y < n; y += 64) {
+                acc = acc + r[y];
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) {
+                const uint32_t x = (uint32_t)(x0 + orow);
+                proj_out[x] = acc;
+                if (slot != 0xFFFFFFFFu) {
+                    prof_hist[(size_t)slot * n + x] = acc;
+                }
+            }
+        }
+    }
+    VFPS_STAMP(7);
+    // ---- H: the last tile closes the step (direct form) ----------------------
+    if (DIRECT) {
+        if (tid == 0) {
+            if (csr) {
+                d.csr_part[tile] = csr_q;
+            }
+            __threadfence();
+            const uint32_t t = atomicAdd(d.ticket, 1u);
+            last_sh = (t == d.tiles - 1u) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (last_sh) {
+            __threadfence();
+            // CSR power: shares summed in tile order
+            double p = 0.0;
+            if (csr) {
+                double a = 0.0;
+                for (uint32_t k = tid; k < d.tiles; k += bs) {
+                    a += d.csr_part[k];
+                }
+                p = block_sum(a, red);
+            }
+            // FP64 charge integral of the profile that entered the step
+            // (src/PS/PhaseSpace.cpp:200-239)
+            double f = 0.0;
+            for (uint32_t x = tid; x < n; x += bs) {
+                f += (double)pprev[x] * (double)ws_g[x];
+            }
+            f = block_sum(f, red);
+            if (tid == 0) {
+                if (csr) {
+                    *d.csr_power = (float)p;
+                    if (d.csr_hist && cnt < d.hist_cap) {
+                        d.csr_hist[cnt] = (float)p;
+                    }
+                }
+                if (d.filling) {
+                    d.filling[0] = f;
+                    d.filling[1] = f;
+                }
+                if (d.counter) {
+                    *d.counter = cnt + 1u;
+                }
+                *d.ticket = 0u;
+            }
+        }
+    }
+    if (d.tstamp) {
+        // per-workgroup entry and exit times
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            d.tstamp[16 + 2 * logical] = t_entry;
+            d.tstamp[17 + 2 * logical] = wall_clock64();
+        }
+    }
+}
+
+#undef VFPS_STAMP
+
+}  // namespace vfps_impl

@@ -19,7 +19,7 @@ __all__ = ["Device", "PhaseSpace", "KickMap", "DriftMap", "RFKickMap",
            "ConstImpedance", "CollimatorImpedance", "makeImpedance",
            "ElectricField", "Step", "derive", "airy", "VfpsError",
            "VfpsInvalid", "InterpolationType", "Axis", "FPType",
-           "DerivationType"]
+           "DerivationType", "toeplitz_kernels", "Batch"]
 
 
 class InterpolationType:  # inc/SM/SourceMap.hpp:22-27
@@ -505,22 +505,43 @@ def derive(**ring):
     return out
 
 
+def toeplitz_kernels(nmax, n, z=None, zc=None, renorm=None, fdelta=0.0):
+    """FP64 Toeplitz kernels (g, h) of the direct wake and CSR forms, index
+    d + n - 1 for |d| < n (vfps_toeplitz_kernels; host only)."""
+    g = np.zeros(2 * n - 1, np.float64) if z is not None else None
+    h = np.zeros(2 * n - 1, np.float64) if zc is not None else None
+    zz = np.ascontiguousarray(z, np.complex64) if z is not None else None
+    zcc = np.ascontiguousarray(zc, np.complex64) if zc is not None else None
+    rn = _f32(renorm) if renorm is not None else None
+    call("vfps_toeplitz_kernels", nmax, n,
+         _p(zz) if zz is not None else None,
+         _p(zcc) if zcc is not None else None,
+         _p(rn) if rn is not None else None, float(fdelta),
+         _p(g) if g is not None else None, _p(h) if h is not None else None)
+    return g, h
+
+
+def _step_desc(desc, out=None):
+    d = out if out is not None else StepDesc()
+    d.interp, d.fused, d.nslip, d.use_fp, d.fptype = 4, 1, 3, 1, 3
+    d.deriv = 4
+    for k, v in desc.items():
+        if k == "slip":
+            s = list(v) + [0.0] * (3 - len(v))
+            d.slip = (C.c_float * 3)(*s)
+            d.nslip = len(v)
+        else:
+            setattr(d, k, v)
+    return d
+
+
 class Step:
     """The time loop of src/main.cpp:948-1053 on one mesh (see
     vfps_step_desc in include/vfps_gpu.h for the fields of desc)."""
 
     def __init__(self, desc, init, zwake=None, zcsr=None, dev=None):
         self.dev = dev or Device.default()
-        d = StepDesc()
-        d.interp, d.fused, d.nslip, d.use_fp, d.fptype = 4, 1, 3, 1, 3
-        d.deriv = 4
-        for k, v in desc.items():
-            if k == "slip":
-                s = list(v) + [0.0] * (3 - len(v))
-                d.slip = (C.c_float * 3)(*s)
-                d.nslip = len(v)
-            else:
-                setattr(d, k, v)
+        d = _step_desc(desc)
         self.desc = d
         self.n = d.n
         init = _f32(init).reshape(self.n, self.n)
@@ -566,13 +587,113 @@ class Step:
         return a.value, b.value, c.value
 
     def phases(self):
-        """Per-phase end times (ns) of one fused step: (maps, field)."""
-        out = np.zeros(16, np.float64)
-        call("vfps_step_phases", self.h, _p(out), 16)
-        return out[1:7].copy(), out[9:14].copy()
+        """Device timestamps of one fused step (ns): the step kernel's phase
+        ends in its first tile, the field kernel's phase ends (FFT wake
+        only), and the step workgroups' (gap after the field kernel, entry
+        spread, shortest, median, longest, last exit)."""
+        out = np.zeros(24, np.float64)
+        call("vfps_step_phases", self.h, _p(out), 24)
+        return out[1:8].copy(), out[9:14].copy(), out[16:22].copy()
+
+    def config(self):
+        """The configuration in use: fused, wake method (1 FFT, 2 direct),
+        tile rows, threads, LDS buffer rows, LDS bytes."""
+        out = np.zeros(6, np.uint32)
+        call("vfps_step_config", self.h, _p(out), 6)
+        return dict(zip(("fused", "wake_method", "tile_rows", "threads",
+                         "stride", "lds_bytes"), (int(v) for v in out)))
 
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
             lib.vfps_step_destroy(h)
+            self.h = None
+
+
+class Batch:
+    """Independent meshes of any sizes advanced together, one launch per
+    step (vfps_batch_* in include/vfps_gpu.h).  descs, inits, zwake, zcsr:
+    one entry per instance."""
+
+    def __init__(self, descs, inits, zwake, zcsr, dev=None):
+        self.dev = dev or Device.default()
+        k = len(descs)
+        arr = (StepDesc * k)()
+        for i, dsc in enumerate(descs):
+            _step_desc(dsc, arr[i])
+        self.n = [int(dsc["n"]) for dsc in descs]
+        self._inits = [_f32(m).reshape(self.n[i], self.n[i])
+                       for i, m in enumerate(inits)]
+        ptrs = (C.c_void_p * k)(*[m.ctypes.data for m in self._inits])
+        zw = (C.c_void_p * k)(*[z.h if z else None for z in zwake])
+        zc = (C.c_void_p * k)(*[z.h if z else None for z in zcsr])
+        h = C.c_void_p()
+        call("vfps_batch_create", self.dev.h, k, arr, ptrs, zw, zc,
+             C.byref(h))
+        self.h = h
+        self._keep = (list(zwake), list(zcsr))
+        self.cap = 0
+
+    def __len__(self):
+        return len(self.n)
+
+    def run(self, nsteps):
+        call("vfps_batch_run", self.h, nsteps)
+
+    def record(self, capacity, profiles=False):
+        call("vfps_batch_record", self.h, capacity, int(profiles))
+        self.cap = capacity
+        self.profiles = profiles
+
+    def history(self, i):
+        cnt = C.c_uint32()
+        csr = np.zeros(self.cap, np.float32)
+        prof = np.zeros((self.cap, self.n[i]), np.float32) if getattr(
+            self, "profiles", False) else None
+        call("vfps_batch_get_history", self.h, i, C.byref(cnt), _p(csr),
+             _p(prof) if prof is not None else None)
+        c = cnt.value
+        return csr[:c], (prof[:c] if prof is not None else None)
+
+    def mesh(self, i):
+        n = self.n[i]
+        m = np.empty((n, n), np.float32)
+        p = np.empty(n, np.float32)
+        w = np.empty(n, np.float32)
+        call("vfps_batch_get_mesh", self.h, i, _p(m), _p(p), _p(w))
+        return m, p, w
+
+    def info(self):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        call("vfps_batch_info", self.h, C.byref(a), C.byref(b), C.byref(c))
+        return dict(instances=a.value, launch_groups=b.value,
+                    cells=c.value)
+
+    def config(self, i):
+        out = np.zeros(6, np.uint32)
+        call("vfps_batch_config", self.h, i, _p(out), 6)
+        return dict(zip(("fused", "wake_method", "tile_rows", "threads",
+                         "stride", "lds_bytes"), (int(v) for v in out)))
+
+    def timing(self, nsteps):
+        ms = C.c_float()
+        call("vfps_batch_timing", self.h, nsteps, C.byref(ms))
+        return ms.value
+
+    def autotune(self, steps=20):
+        """Pick the tile height and workgroup size per mesh size; returns the
+        timing table (n, T, threads, us/step, chosen)."""
+        cap = 64
+        table = np.zeros((cap, 5), np.float32)
+        rows = C.c_uint32()
+        call("vfps_batch_autotune", self.h, steps, _p(table), cap,
+             C.byref(rows))
+        return [dict(n=int(r[0]), T=int(r[1]), threads=int(r[2]),
+                     us_per_step=float(r[3]), chosen=bool(r[4]))
+                for r in table[:rows.value]]
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib.vfps_batch_destroy(h)
             self.h = None

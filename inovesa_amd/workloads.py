@@ -110,3 +110,42 @@ def impedances(d, dev=None, resistive_wall=None):
     zc = makeImpedance(d["padded_bins"], d["fmax"], d["R_bend"],
                        ring["f_rev"], gap if gap > 0 else -1.0, dev=dev)
     return zw, zc
+
+
+def batched_scan_config(sizes=(128, 256, 384, 512), per_size=16, seed=3,
+                        current_range=(0.3, 3.0), gap_range=(0.016, 0.064),
+                        steps_per_ts=1000, damping_steps=1000.0):
+    """Independent instances of uneven mesh sizes for a threshold scan:
+    per_size instances of each size (sizes interleaved), currents
+    log-spaced over current_range times each instance's scaling-law
+    threshold, parallel-plates gaps drawn uniformly from gap_range (m) with
+    the given seed, and the rotation scheme (cubic clamped RotationMap,
+    FokkerPlanckMap with damping time damping_steps steps, wake kick) with
+    each instance's own parallel-plates CSR impedance.  Returns a list of
+    (desc, derived, init) per instance."""
+    k = len(sizes) * per_size
+    rng = np.random.Generator(np.random.Philox(seed))
+    gaps = rng.uniform(gap_range[0], gap_range[1], k)
+    factors = np.geomspace(current_range[0], current_range[1], k)
+    out = []
+    for i in range(k):
+        n = sizes[i % len(sizes)]
+        base = derive(grid_size=n, steps_per_ts=steps_per_ts,
+                      vacuum_gap=float(gaps[i]))
+        d = derive(grid_size=n, steps_per_ts=steps_per_ts,
+                   vacuum_gap=float(gaps[i]),
+                   damping_time=damping_steps * base["dt"],
+                   bunch_current=float(factors[i]) * base["Ith"])
+        desc = dict(n=n, qmin=d["qmin"], qmax=d["qmax"], pmin=d["pmin"],
+                    pmax=d["pmax"], qscale=d["qscale"], pscale=d["pscale"],
+                    scheme=1, interp=4, rot_clamp=1, angle=d["angle"],
+                    rf_linear=1, rf_angle=d["angle"],
+                    revolutionpart=d["revolutionpart"], V_RF=d["V_eff"],
+                    f_RF=d["f_RF"], V0=d["V0"], slip=d["slip"],
+                    E0=d["ring"]["beam_energy"], use_fp=1, fptype=3,
+                    e1=d["e1"], deriv=4, use_wake=1,
+                    spacing_bins=d["spacing_bins"],
+                    wakescaling=d["wakescaling"], use_csr=1,
+                    fc=d["ring"]["cutoff_frequency"])
+        out.append((desc, d, initial_mesh(n, seed=seed * 1000 + i)))
+    return out

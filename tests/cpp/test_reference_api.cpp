@@ -1,0 +1,365 @@
+// The reference's unit tests (test/TestKickMap.cpp, test/TestRotationMap.cpp,
+// test/TestPhaseSpace.cpp, test/TestElectricField.cpp, test/TestImpedance.cpp)
+// restated against the C++ interface of include/inovesa_amd.hpp, and the
+// time loop of src/main.cpp:948-1053 written with the same classes, checked
+// against the fused device step.
+//
+// Usage: test_reference_api <fixture file>
+// The fixture file holds the reference test vectors and the loop
+// parameters as whitespace-separated records (written by
+// tests/test_cpp_api.py from tests/golden/reference_vectors.json).
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "inovesa_amd.hpp"
+
+using namespace inovesa_amd;
+
+static int failures = 0;
+static int checks = 0;
+
+#define CHECK(cond)                                                           \
+    do {                                                                      \
+        checks++;                                                             \
+        if (!(cond)) {                                                        \
+            failures++;                                                       \
+            std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);   \
+        }                                                                     \
+    } while (0)
+
+#define CHECK_SMALL(v, tol) CHECK(std::fabs((double)(v)) <= (tol))
+
+// one record: a name, then key value... pairs; arrays are "key n v0 v1 ..."
+struct Record {
+    std::string name;
+    std::map<std::string, std::vector<double>> v;
+    double num(const std::string& k) const { return v.at(k).at(0); }
+    std::vector<float> arr(const std::string& k) const
+    {
+        const auto& a = v.at(k);
+        return std::vector<float>(a.begin(), a.end());
+    }
+};
+
+static std::vector<Record> read_fixture(const char* path)
+{
+    std::ifstream f(path);
+    std::vector<Record> out;
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream is(line);
+        Record r;
+        if (!(is >> r.name)) continue;
+        std::string key;
+        size_t cnt;
+        while (is >> key >> cnt) {
+            std::vector<double> a(cnt);
+            for (size_t i = 0; i < cnt; i++) is >> a[i];
+            r.v[key] = a;
+        }
+        out.push_back(r);
+    }
+    return out;
+}
+
+// test/TestKickMap.cpp:18-172
+static void kickmap_case(const Record& r, oclhptr_t oclh)
+{
+    std::vector<float> buckets{1};
+    PhaseSpace::resetSize(4, 1);
+    auto in = r.arr("data_in");
+    auto expect = r.arr("data_out");
+    auto ps1 = std::make_shared<PhaseSpace>(-1, 1, 2, -1, 1, 4, oclh, 1, 1,
+                                            buckets, 1, in.data());
+    auto ps2 = std::make_shared<PhaseSpace>(-1, 1, 2, -1, 1, 4, oclh, 1, 1,
+                                            buckets, 1);
+    KickMap km(ps1, ps2, (SourceMap::InterpolationType)(int)r.num("it"), false,
+               r.num("axis") == 0 ? KickMap::Axis::x : KickMap::Axis::y, oclh);
+    std::vector<meshaxis_t> offset = r.arr("offset");
+    km.swapOffset(offset);
+    km.apply();
+    const meshdata_t* d = ps2->getData();
+    for (int i = 0; i < 16; i++) {
+        CHECK_SMALL(d[i] - expect[i], 1e-5);
+    }
+    const meshaxis_t* f = km.getForce();
+    for (int i = 0; i < 4; i++) {
+        CHECK(f[i] == r.arr("offset")[i]);
+    }
+}
+
+// test/TestRotationMap.cpp:18-205
+static void rotation_case(const Record& r, oclhptr_t oclh)
+{
+    std::vector<float> buckets{1};
+    PhaseSpace::resetSize(4, 1);
+    auto in = r.arr("data_in");
+    auto expect = r.arr("data_out");
+    auto ps1 = std::make_shared<PhaseSpace>(
+        r.num("qmin"), r.num("qmax"), 1, r.num("pmin"), r.num("pmax"), 1, oclh,
+        1, 1, buckets, 1, in.data());
+    auto ps2 = std::make_shared<PhaseSpace>(*ps1);
+    RotationMap rm(ps1, ps2, 4, 4, (meshaxis_t)r.num("angle"),
+                   (SourceMap::InterpolationType)(int)r.num("it"),
+                   r.num("clamp") != 0, (meshindex_t)r.num("rotmapsize"), oclh);
+    rm.apply();
+    const meshdata_t* d = ps2->getData();
+    for (int i = 0; i < 16; i++) {
+        CHECK_SMALL(d[i] - expect[i], 1e-5);
+    }
+}
+
+// src/SM/RotationMap.cpp:83-87: clamping with linear interpolation throws
+static void rotation_throws_case(const Record& r, oclhptr_t oclh)
+{
+    PhaseSpace::resetSize(4, 1);
+    auto ps1 = std::make_shared<PhaseSpace>(-1, 1, 1, -1, 1, 1, oclh, 1, 1);
+    auto ps2 = std::make_shared<PhaseSpace>(*ps1);
+    bool threw = false;
+    try {
+        RotationMap rm(ps1, ps2, 4, 4, 0.5f,
+                       (SourceMap::InterpolationType)(int)r.num("it"),
+                       r.num("clamp") != 0, (meshindex_t)r.num("rotmapsize"),
+                       oclh);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+// test/TestPhaseSpace.cpp:38-107: integral and width of the Gaussian
+static void phasespace_case(const Record& r, oclhptr_t oclh)
+{
+    const meshindex_t n = (meshindex_t)r.num("n");
+    PhaseSpace::resetSize(n, 1);
+    PhaseSpace ps(r.num("min"), r.num("max"), 1, r.num("min"), r.num("max"), 1,
+                  oclh, 1, 1);
+    ps.updateXProjection();
+    ps.integrate();
+    CHECK_SMALL(ps.getIntegral() - r.num("integral"),
+                r.num("rel_tol") * r.num("integral"));
+    ps.variance(0);
+    CHECK_SMALL(ps.getBunchLength() - r.num("sigma"),
+                r.num("rel_tol") * r.num("sigma"));
+    // the copy holds the same data
+    PhaseSpace copy(ps);
+    const std::vector<float> a(ps.getData(), ps.getData() + n * n);
+    const meshdata_t* b = copy.getData();
+    bool same = true;
+    for (size_t i = 0; i < a.size(); i++) same = same && a[i] == b[i];
+    CHECK(same);
+}
+
+// test/TestElectricField.cpp:110-239
+static void electric_field_cases(oclhptr_t oclh)
+{
+    const meshindex_t n = 32;
+    const meshindex_t spacing = 256;
+    PhaseSpace::resetSize(n, 1);
+    auto ps = std::make_shared<PhaseSpace>(-12, 12, 3, -12, 12, 4, oclh, 1, 1);
+    ps->updateXProjection();
+    auto z = std::make_shared<ConstImpedance>(spacing, 1e9f, impedance_t(1, 0),
+                                              oclh);
+    ElectricField f(ps, z, {0}, spacing, oclh, 1e6, 0.1f, 1.0f);
+    // padded profile: the projection, then zeros
+    f.padBunchProfiles();
+    const float* pad = f.getPaddedBunchProfiles();
+    const std::vector<float> proj = ps->getProjection(0);
+    for (meshindex_t i = 0; i < n; i++) CHECK(pad[i] == proj[i]);
+    for (meshindex_t i = n; i < spacing; i++) CHECK(pad[i] == 0.0f);
+    // a constant impedance makes the wake the scaled profile
+    const meshaxis_t* wp = f.wakePotential();
+    const float s = f.getWakeScaling();
+    for (meshindex_t i = 0; i < n; i++) {
+        CHECK_SMALL(wp[i] - proj[i] * s * spacing, 1e-6);
+    }
+    // the forward wake of a local change does not reach earlier bins
+    auto zc = std::make_shared<FreeSpaceCSR>(spacing, 1e6f, 1e12f, oclh);
+    ElectricField fc(ps, zc, {0}, spacing, oclh, 1e6, 0.1f, 1.0f);
+    const std::vector<float> w0(fc.wakePotential(), fc.wakePotential() + n);
+    std::vector<float> p2 = proj;
+    const meshindex_t c = n / 2;
+    const float dl = 0.1f * p2[c];
+    p2[c] -= dl;
+    p2[c + 1] += dl;
+    p2[c + 2] += dl;
+    p2[c + 3] -= dl;
+    ps->setProjection(0, p2);
+    const meshaxis_t* w1 = fc.wakePotential();
+    float early = 0, late = 0;
+    for (meshindex_t i = 0; i < n; i++) {
+        const float dd = std::fabs(w1[i] - w0[i]);
+        if (i < c) early = std::max(early, dd);
+        else late = std::max(late, dd);
+    }
+    CHECK(early < 2e-8f);
+    CHECK(late > 2e-8f);
+}
+
+// test/TestImpedance.cpp:48-86
+static void impedance_cases(const Record& add, const Record& cst, oclhptr_t oclh)
+{
+    auto z1v = add.arr("z1");
+    auto z2v = add.arr("z2");
+    std::vector<impedance_t> z1, z2;
+    for (size_t i = 0; i + 1 < z1v.size(); i += 2) z1.emplace_back(z1v[i], z1v[i + 1]);
+    for (size_t i = 0; i + 1 < z2v.size(); i += 2) z2.emplace_back(z2v[i], z2v[i + 1]);
+    Impedance a(z1, 1e9f, oclh);
+    Impedance b(z2, 1e9f, oclh);
+    a += b;
+    const auto& s = a.impedance();
+    for (size_t i = 0; i < z1.size(); i++) {
+        CHECK(s[i] == z1[i] + z2[i]);
+    }
+    const size_t n = (size_t)cst.num("n");
+    const auto zz = cst.arr("z");
+    ConstImpedance c(n, 1e9f, impedance_t(zz[0], zz[1]), oclh);
+    const auto& cv = c.impedance();
+    const auto e = cst.arr("expect");
+    for (size_t i = 0; i < n; i++) {
+        CHECK(cv[i] == impedance_t(e[2 * i], e[2 * i + 1]));
+    }
+    // factory: nothing selected gives nullptr; a gap selects the plates
+    CHECK(!makeImpedance(1024, oclh, 1e12f, 5.559, 9e6, 0.0));
+    auto pp = makeImpedance(1024, oclh, 1e12f, 5.559, 9e6, 0.032);
+    CHECK(pp && pp->nFreqs() == 1024 && pp->impedance()[0] == impedance_t(0, 0));
+}
+
+// The time loop of src/main.cpp:948-1053 with the reference-shaped classes
+// (wake update, integral, CSR, wake kick t1->t2, RF kick t2->t1, drift
+// t1->t3, Fokker-Planck t3->t1, x projection), against the fused device
+// step with the FFT wake: equal bit for bit.
+static void time_loop_case(const Record& r, oclhptr_t oclh)
+{
+    const meshindex_t n = (meshindex_t)r.num("n");
+    const int steps = (int)r.num("steps");
+    PhaseSpace::resetSize(n, 1);
+    const std::vector<float> init = r.arr("init");
+    auto t1 = std::make_shared<PhaseSpace>(
+        r.num("qmin"), r.num("qmax"), r.num("qscale"), r.num("pmin"),
+        r.num("pmax"), r.num("pscale"), oclh, 1, 1, std::vector<float>{1}, 1,
+        init.data());
+    auto t2 = std::make_shared<PhaseSpace>(*t1);
+    auto t3 = std::make_shared<PhaseSpace>(*t1);
+    t1->updateXProjection();
+    const size_t nmax = (size_t)r.num("nmax");
+    std::shared_ptr<Impedance> zw(new ParallelPlatesCSR(
+        nmax, (frequency_t)r.num("f0"), (frequency_t)r.num("fmax"),
+        r.num("gap"), oclh));
+    ElectricField field(t1, zw, {0}, (meshindex_t)r.num("spacing_bins"), oclh,
+                        1.0, 1.0f, (meshaxis_t)r.num("wakescaling"));
+    const auto it = SourceMap::cubic;
+    WakePotentialMap wkm(t1, t2, &field, it, false, oclh);
+    RFKickMap rfm(t2, t1, (timeaxis_t)r.num("revolutionpart"),
+                  (meshaxis_t)r.num("V_RF"), (frequency_t)r.num("f_RF"),
+                  (meshaxis_t)r.num("V0"), it, false, oclh);
+    const std::vector<meshaxis_t> slip = r.arr("slip");
+    DriftMap drm(t1, t3, slip, (meshaxis_t)r.num("E0"), it, false, oclh);
+    FokkerPlanckMap fpm(t3, t1, n, n, FokkerPlanckMap::FPType::full,
+                        FokkerPlanckMap::FPTracking::none,
+                        (timeaxis_t)r.num("e1"),
+                        FokkerPlanckMap::DerivationType::cubic, oclh);
+    for (int k = 0; k < steps; k++) {
+        wkm.update();
+        t1->integrate();
+        wkm.apply();
+        rfm.apply();
+        drm.apply();
+        fpm.apply();
+        t1->updateXProjection();
+    }
+    const std::vector<float> loop_mesh(t1->getData(), t1->getData() + n * n);
+    const std::vector<float> loop_proj = t1->getProjection(0);
+
+    // the fused step
+    vfps_step_desc d{};
+    d.n = n;
+    d.qmin = (float)r.num("qmin");
+    d.qmax = (float)r.num("qmax");
+    d.pmin = (float)r.num("pmin");
+    d.pmax = (float)r.num("pmax");
+    d.qscale = r.num("qscale");
+    d.pscale = r.num("pscale");
+    d.scheme = 0;
+    d.interp = 4;
+    d.revolutionpart = (float)r.num("revolutionpart");
+    d.V_RF = (float)r.num("V_RF");
+    d.f_RF = (float)r.num("f_RF");
+    d.V0 = (float)r.num("V0");
+    for (size_t i = 0; i < slip.size() && i < 3; i++) d.slip[i] = slip[i];
+    d.nslip = (uint32_t)slip.size();
+    d.E0 = (float)r.num("E0");
+    d.use_fp = 1;
+    d.fptype = 3;
+    d.e1 = (float)r.num("e1");
+    d.deriv = 4;
+    d.use_wake = 1;
+    d.spacing_bins = (uint32_t)r.num("spacing_bins");
+    d.wakescaling = (float)r.num("wakescaling");
+    d.fused = 1;
+    d.wake_method = 1;
+    vfps_step* st = nullptr;
+    check(vfps_step_create(oclh->handle(), &d, init.data(), zw->handle(),
+                           nullptr, &st));
+    check(vfps_step_run(st, (uint32_t)steps));
+    std::vector<float> mesh((size_t)n * n), proj(n);
+    check(vfps_step_get_mesh(st, mesh.data(), proj.data(), nullptr));
+    vfps_step_destroy(st);
+    bool same = true;
+    for (size_t i = 0; i < mesh.size(); i++) same = same && mesh[i] == loop_mesh[i];
+    for (size_t i = 0; i < n; i++) same = same && proj[i] == loop_proj[i];
+    CHECK(same);
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s <fixture>\n", argv[0]);
+        return 2;
+    }
+    const auto recs = read_fixture(argv[1]);
+    auto oclh = std::make_shared<Device>(0);
+    std::map<std::string, const Record*> named;
+    for (const auto& r : recs) named[r.name] = &r;
+    try {
+        for (const auto& r : recs) {
+            const int before = failures;
+            if (r.name.rfind("kickmap", 0) == 0) {
+                kickmap_case(r, oclh);
+            } else if (r.name.rfind("rotation_throws", 0) == 0) {
+                rotation_throws_case(r, oclh);
+            } else if (r.name.rfind("rotation", 0) == 0) {
+                rotation_case(r, oclh);
+            } else if (r.name == "phasespace_gaussian") {
+                phasespace_case(r, oclh);
+            } else if (r.name == "time_loop") {
+                time_loop_case(r, oclh);
+            } else {
+                continue;
+            }
+            std::printf("%s %s\n", r.name.c_str(),
+                        failures == before ? "ok" : "FAILED");
+        }
+        int before = failures;
+        electric_field_cases(oclh);
+        std::printf("electric_field %s\n", failures == before ? "ok" : "FAILED");
+        before = failures;
+        impedance_cases(*named.at("impedance_addition"),
+                        *named.at("const_impedance"), oclh);
+        std::printf("impedance %s\n", failures == before ? "ok" : "FAILED");
+    } catch (const std::exception& e) {
+        std::printf("exception: %s\n", e.what());
+        return 1;
+    }
+    std::printf("%d checks, %d failures\n", checks, failures);
+    if (failures == 0) {
+        std::printf("ALL PASSED\n");
+    }
+    return failures == 0 ? 0 : 1;
+}

@@ -1,0 +1,103 @@
+"""Batched ragged step (many independent meshes of uneven sizes, one launch
+per step) and its autotuner.  Pinned by the single-mesh step: every
+instance of a batch must equal the same instance run alone, bit for bit,
+whatever tiling the autotuner picks."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def V():
+    import inovesa_amd as V
+    return V
+
+
+def _instances(V, sizes, per_size, kick_drift=False, seed=3):
+    from inovesa_amd.workloads import (batched_scan_config, impedances,
+                                       kick_drift_config, initial_mesh)
+    cfg = batched_scan_config(sizes=sizes, per_size=per_size, seed=seed)
+    out = []
+    for desc, d, init in cfg:
+        zw, zc = impedances(d)
+        out.append((desc, zw, zc, init))
+    if kick_drift:
+        desc, d = kick_drift_config(n=256, steps_per_ts=500)
+        zw, zc = impedances(d)
+        out.append((desc, zw, zc, initial_mesh(256, seed=11)))
+    return out
+
+
+def _batch(V, inst):
+    return V.Batch([i[0] for i in inst], [i[3] for i in inst],
+                   [i[1] for i in inst], [i[2] for i in inst])
+
+
+def _single(V, i):
+    desc, zw, zc, init = i
+    return V.Step(dict(desc, wake_method=2), init, zw, zc)
+
+
+@pytest.mark.parametrize("kick_drift", [False, True])
+def test_batch_equals_single_steps(V, kick_drift):
+    inst = _instances(V, (128, 256, 384), 2, kick_drift)
+    b = _batch(V, inst)
+    info = b.info()
+    assert info["instances"] == len(inst)
+    assert info["launch_groups"] == (2 if kick_drift else 1)
+    steps = 70                       # one graph replay and 6 single steps
+    b.record(steps + 8)
+    b.run(steps)
+    for k, i in enumerate(inst):
+        s = _single(V, i)
+        s.record(steps + 8)
+        s.run(steps)
+        for x, y in zip(b.mesh(k), s.mesh()):
+            assert np.array_equal(x, y), k
+        cb, _ = b.history(k)
+        cs, _ = s.history()
+        assert len(cb) == steps and np.array_equal(cb, cs), k
+
+
+def test_autotune_keeps_results_and_reports_table(V):
+    inst = _instances(V, (128, 256, 384, 512), 1)
+    a = _batch(V, inst)
+    b = _batch(V, inst)
+    a.run(10)
+    b.run(10)
+    table = a.autotune(steps=6)
+    sizes = sorted({r["n"] for r in table})
+    assert sizes == [128, 256, 384, 512]
+    for n in sizes:
+        rows = [r for r in table if r["n"] == n]
+        assert len(rows) >= 3
+        chosen = [r for r in rows if r["chosen"]]
+        assert len(chosen) == 1
+        assert chosen[0]["us_per_step"] == min(r["us_per_step"] for r in rows)
+        k = [j for j, i in enumerate(inst) if i[0]["n"] == n][0]
+        cfg = a.config(k)
+        assert (cfg["tile_rows"], cfg["threads"]) == (chosen[0]["T"],
+                                                      chosen[0]["threads"])
+    print("\nautotune table (n, T, threads, us/step, chosen):")
+    for r in table:
+        print(f"  {r['n']:4d} {r['T']:2d} {r['threads']:4d} "
+              f"{r['us_per_step']:8.2f} {'*' if r['chosen'] else ''}")
+    # tuning restored the state: both batches continue identically
+    a.run(66)
+    b.run(66)
+    for k in range(len(inst)):
+        for x, y in zip(a.mesh(k), b.mesh(k)):
+            assert np.array_equal(x, y), k
+
+
+def test_batch_timing_and_rejects(V):
+    inst = _instances(V, (128,), 2)
+    b = _batch(V, inst)
+    ms = b.timing(8)
+    assert 0 < ms < 10
+    with pytest.raises(V.VfpsInvalid):
+        b.mesh(5)
+    bad = dict(inst[0][0], scheme=7)
+    with pytest.raises(V.VfpsInvalid):
+        V.Batch([bad], [inst[0][3]], [inst[0][1]], [inst[0][2]])

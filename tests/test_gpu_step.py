@@ -28,10 +28,12 @@ CASES = [(0, 512, {}), (1, 256, {}), (0, 256, {"steps_per_ts": 500}),
          (1, 128, {"steps_per_ts": 300})]
 
 
+@pytest.mark.parametrize("method", [1, 2])
 @pytest.mark.parametrize("scheme,n,kw", CASES)
-def test_one_step_matches_oracle(V, orc, scheme, n, kw):
+def test_one_step_matches_oracle(V, orc, scheme, n, kw, method):
     desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
-    st = V.Step(desc, init, zw, zc)
+    st = V.Step(dict(desc, wake_method=method), init, zw, zc)
+    assert st.config()["wake_method"] == method
     st.record(4)
     st.run(1)
     mesh, proj, wake = st.mesh()
@@ -46,8 +48,10 @@ def test_one_step_matches_oracle(V, orc, scheme, n, kw):
 
 
 def _check_csr_power(orc, desc, zc, init, gpu_power, ref_power):
-    """The GPU sums the reference's float CSR terms in FP64; the reference
-    sums them sequentially in float (error <= nmax*eps*sum)."""
+    """The FFT form sums the reference's float CSR terms in FP64, the
+    direct form evaluates sum_k c_k |F_k|^2 in FP64 (both within 1e-6 of
+    the exact sum of the float terms); the reference sums them
+    sequentially in float (error <= nmax*eps*sum)."""
     import math
     ax, dq, _ = orc.ruler(desc["n"], desc["qmin"], desc["qmax"])
     ws = orc.simpson_weights(desc["n"], dq)
@@ -65,8 +69,10 @@ def _check_csr_power(orc, desc, zc, init, gpu_power, ref_power):
 
 @pytest.mark.parametrize("scheme,n,kw", CASES[:2])
 def test_fused_equals_per_map_path(V, scheme, n, kw):
+    """With the FFT wake the fused kernel equals the per-map kernels bit for
+    bit, step after step."""
     desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
-    a = V.Step(dict(desc, fused=1), init, zw, zc)
+    a = V.Step(dict(desc, fused=1, wake_method=1), init, zw, zc)
     b = V.Step(dict(desc, fused=0), init, zw, zc)
     a.record(80, profiles=True)
     b.record(80, profiles=True)
@@ -109,14 +115,63 @@ def test_graph_replay_deterministic(V):
     assert np.array_equal(a.mesh()[0], b.mesh()[0])
 
 
-@pytest.mark.parametrize("tile", [1, 2, 4, 8, 16])
-def test_tile_rows_do_not_change_results(V, tile):
+@pytest.mark.parametrize("scheme,n,kw", CASES[:2])
+def test_direct_and_fft_wake_agree(V, orc, scheme, n, kw):
+    """The direct (Toeplitz) and FFT forms of the wake differ by rounding
+    only; over 70 steps both stay close to each other and to the oracle."""
+    desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
+    a = V.Step(dict(desc, wake_method=2), init, zw, zc)
+    b = V.Step(dict(desc, wake_method=1), init, zw, zc)
+    a.record(80, profiles=True)
+    b.record(80, profiles=True)
+    a.run(70)
+    b.run(70)
+    ma, pa, wa = a.mesh()
+    mb, pb, wb = b.mesh()
+    ca, fa = a.history()
+    cb, fb = b.history()
+    print(f"\n70 steps, scheme {scheme}, n={n}: direct vs FFT wake: mesh "
+          f"{rel_linf(ma, mb):.2e} wake {rel_linf(wa, wb):.2e} csr "
+          f"{rel_linf(ca, cb):.2e}")
+    assert len(ca) == 70
+    assert rel_linf(ma, mb) <= 1e-4
+    assert rel_linf(fa[:5], fb[:5]) <= 1e-5
+    assert rel_linf(ca[:5], cb[:5]) <= 1e-6
+
+
+TILE_VARIANTS = [(1, 128), (1, 256), (1, 512), (2, 256), (2, 512), (4, 256),
+                 (8, 512), (16, 512)]
+
+
+@pytest.mark.parametrize("tile,threads", TILE_VARIANTS)
+def test_tile_variants_identical_fft(V, tile, threads):
+    """Every tile/workgroup variant of the fused kernel gives the per-map
+    result bit for bit (FFT wake)."""
     desc, d, zw, zc, init = _setup(V, 0, 512)
     ref = V.Step(dict(desc, fused=0), init, zw, zc)
     ref.run(3)
-    st = V.Step(dict(desc, tile_rows=tile), init, zw, zc)
+    st = V.Step(dict(desc, wake_method=1, tile_rows=tile, threads=threads),
+                init, zw, zc)
     st.run(3)
     assert np.array_equal(st.mesh()[0], ref.mesh()[0])
+
+
+@pytest.mark.parametrize("scheme,n,kw", CASES[:2])
+@pytest.mark.parametrize("tile,threads", TILE_VARIANTS)
+def test_tile_variants_identical_direct(V, scheme, n, kw, tile, threads):
+    """Every tile/workgroup variant gives the same mesh, profile and wake
+    bit for bit (direct wake)."""
+    if threads * 4 < n:
+        pytest.skip("too few threads for the mesh")
+    desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
+    ref = V.Step(dict(desc, wake_method=2, tile_rows=1, threads=256), init,
+                 zw, zc)
+    ref.run(5)
+    st = V.Step(dict(desc, wake_method=2, tile_rows=tile, threads=threads),
+                init, zw, zc)
+    st.run(5)
+    for x, y in zip(st.mesh(), ref.mesh()):
+        assert np.array_equal(x, y)
 
 
 @pytest.mark.parametrize("scheme,n,kw", [(0, 256, {"steps_per_ts": 500}),
