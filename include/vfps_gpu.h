@@ -408,6 +408,50 @@ VFPS_API int vfps_batch_timing(vfps_batch* b, uint32_t nsteps, float* ms_step);
 VFPS_API int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                                  uint32_t cap, uint32_t* nrows);
 
+/* ---- learned surrogate (multilayer perceptron) --------------------------- *
+ * No reference counterpart.  A fully connected network dims[0] -> dims[1]
+ * -> ... -> dims[ndims-1] with GELU after every hidden layer, trained and
+ * evaluated on the device: the products run on MFMA (BF16 operands, FP32
+ * accumulation); master weights, gradients and Adam moments are FP32.
+ * Outputs: dims[ndims-1]-1 regression values (mean squared error) and one
+ * logit (binary cross entropy, weight label_weight).  Parameters are
+ * flattened per layer as W [out][in] (row-major) then b [out]. */
+VFPS_API int vfps_mlp_create(vfps_dev* dev, const uint32_t* dims,
+                             uint32_t ndims, uint32_t max_batch,
+                             uint64_t seed, vfps_mlp** out);
+VFPS_API int vfps_mlp_destroy(vfps_mlp* m);
+VFPS_API int vfps_mlp_nparams(const vfps_mlp* m, size_t* n);
+/* set the FP32 parameters (their BF16 copy follows); resets Adam */
+VFPS_API int vfps_mlp_set_params(vfps_mlp* m, const float* params, size_t n);
+/* parameters and the gradients of the last vfps_mlp_loss_grad (either may
+ * be NULL) */
+VFPS_API int vfps_mlp_get_params(vfps_mlp* m, float* params, float* grads,
+                                 size_t n);
+/* y [B][dims_out] = network(x [B][dims_in]) */
+VFPS_API int vfps_mlp_forward(vfps_mlp* m, const float* x, uint32_t B,
+                              float* y);
+/* loss of one batch and its gradients (kept on the device) */
+VFPS_API int vfps_mlp_loss_grad(vfps_mlp* m, const float* x,
+                                const float* target, const float* label,
+                                uint32_t B, float label_weight, float* loss);
+/* one Adam step with the current gradients */
+VFPS_API int vfps_mlp_adam_step(vfps_mlp* m, float lr, float beta1,
+                                float beta2, float eps);
+/* epochs of shuffled minibatch Adam (beta1 0.9, beta2 0.999, eps 1e-8) on N
+ * samples held on the device; epoch_loss (epochs) may be NULL */
+VFPS_API int vfps_mlp_train(vfps_mlp* m, const float* x, const float* target,
+                            const float* label, uint32_t N, uint32_t batch,
+                            uint32_t epochs, float lr, float label_weight,
+                            uint64_t seed, float* epoch_loss);
+/* the layers' product alone: C [M][N] = A B, A(i,k) = A[i*asi + k*ask],
+ * B(k,j) = B[k*bsk + j*bsj], operands rounded to BF16 */
+VFPS_API int vfps_gemm_bf16(vfps_dev* dev, uint32_t M, uint32_t N, uint32_t K,
+                            const float* A, size_t a_len, int64_t asi,
+                            int64_t ask, const float* B, size_t b_len,
+                            int64_t bsk, int64_t bsj, float* C);
+/* the BF16 rounding of the kernels (round to nearest even), host only */
+VFPS_API int vfps_round_bf16(const float* x, float* y, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -168,6 +168,19 @@ _SIGS = {
     "vfps_batch_config": (_int, [_vp, _u32, _vp, _u32]),
     "vfps_batch_timing": (_int, [_vp, _u32, _vp]),
     "vfps_batch_autotune": (_int, [_vp, _u32, _vp, _u32, _vp]),
+    "vfps_mlp_create": (_int, [_vp, _vp, _u32, _u32, C.c_uint64, _vp]),
+    "vfps_mlp_destroy": (_int, [_vp]),
+    "vfps_mlp_nparams": (_int, [_vp, _vp]),
+    "vfps_mlp_set_params": (_int, [_vp, _vp, _sz]),
+    "vfps_mlp_get_params": (_int, [_vp, _vp, _vp, _sz]),
+    "vfps_mlp_forward": (_int, [_vp, _vp, _u32, _vp]),
+    "vfps_mlp_loss_grad": (_int, [_vp, _vp, _vp, _vp, _u32, _f32, _vp]),
+    "vfps_mlp_adam_step": (_int, [_vp, _f32, _f32, _f32, _f32]),
+    "vfps_mlp_train": (_int, [_vp, _vp, _vp, _vp, _u32, _u32, _u32, _f32,
+                              _f32, C.c_uint64, _vp]),
+    "vfps_gemm_bf16": (_int, [_vp, _u32, _u32, _u32, _vp, _sz, C.c_int64,
+                              C.c_int64, _vp, _sz, C.c_int64, C.c_int64, _vp]),
+    "vfps_round_bf16": (_int, [_vp, _vp, _sz]),
     "vfps_toeplitz_kernels": (_int, [_u32, _u32, _vp, _vp, _vp, _f32, _vp,
                                      _vp]),
 }

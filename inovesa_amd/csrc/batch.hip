@@ -296,6 +296,13 @@ int vfps_batch_run(vfps_batch* b, uint32_t nsteps)
         b->parity ^= 1u;
         left--;
     }
+    if (nsteps > 0) {
+        for (auto& g : b->groups) {
+            rc = launch_step_finalize(g.descs_d, (uint32_t)g.members.size(),
+                                      b->parity, st);
+            if (rc != VFPS_OK) return rc;
+        }
+    }
     VFPS_HIP(hipStreamSynchronize(st));
     sync_parity(b);
     for (auto* s : b->members) {
@@ -410,8 +417,8 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                            hipMemcpyDeviceToDevice));
         VFPS_HIP(hipMemcpy(snap[i].proj, cur->proj, sizeof(float) * s->desc.n,
                            hipMemcpyDeviceToDevice));
-        VFPS_HIP(hipMemcpy(&snap[i].counter, s->counter, sizeof(uint32_t),
-                           hipMemcpyDeviceToHost));
+        VFPS_HIP(hipMemcpy(&snap[i].counter, s->counter + (par & 1u),
+                           sizeof(uint32_t), hipMemcpyDeviceToHost));
     }
     std::map<uint32_t, std::vector<uint32_t>> classes;
     for (uint32_t i = 0; i < b->members.size(); i++) {
@@ -508,8 +515,9 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                   hipMemcpyDeviceToDevice);
         hipMemcpy(cur->proj, snap[i].proj, sizeof(float) * s->desc.n,
                   hipMemcpyDeviceToDevice);
-        hipMemcpy(s->counter, &snap[i].counter, sizeof(uint32_t),
-                  hipMemcpyHostToDevice);
+        const uint32_t cc[2] = {snap[i].counter, snap[i].counter};
+        hipMemcpy(s->counter, cc, sizeof(cc), hipMemcpyHostToDevice);
+        hipMemset(s->csr_valid, 0xFF, sizeof(uint32_t) * 2);
         hipFree(snap[i].mesh);
         hipFree(snap[i].proj);
         s->parity = par;

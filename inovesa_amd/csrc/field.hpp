@@ -27,6 +27,8 @@ struct FieldParams {
     float* spectrum;          // optional [nb][2M]
     float* power;             // optional [nb]
     uint32_t* counter;        // optional step counter (history slot)
+    uint32_t counter_pingpong;  // 1: read counter[parity], write [parity^1]
+    uint32_t parity;          // set by the fused step's field kernel
     float* csr_hist;          // optional per-step CSR power history
     uint32_t hist_cap;        // history capacity (steps)
     unsigned long long* tstamp;  // optional phase timestamps (block 0)

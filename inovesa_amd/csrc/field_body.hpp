@@ -112,7 +112,9 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
     if (!WAKE && !CSR) {
         // no impedance: the step only needs the integral and the counter
         if (threadIdx.x == 0 && p.counter) {
-            *p.counter = *p.counter + 1;
+            const uint32_t pin = p.counter_pingpong ? p.parity : 0u;
+            const uint32_t pout = p.counter_pingpong ? (p.parity ^ 1u) : 0u;
+            p.counter[pout] = p.counter[pin] + 1;
         }
         return;
     }
@@ -195,11 +197,13 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
     }
     if (threadIdx.x == 0 && p.counter) {
         // per-step history: this step's CSR power, then count the step
-        const uint32_t c = *p.counter;
+        const uint32_t pin = p.counter_pingpong ? p.parity : 0u;
+        const uint32_t pout = p.counter_pingpong ? (p.parity ^ 1u) : 0u;
+        const uint32_t c = p.counter[pin];
         if (CSR && p.csr_hist && c < p.hist_cap) {
             p.csr_hist[c] = (float)csr;
         }
-        *p.counter = c + 1;
+        p.counter[pout] = c + 1;
     }
     if (!WAKE) {
         return;

@@ -173,6 +173,7 @@ fields_kernel(const FieldParams* __restrict__ params, uint32_t parity)
     if (parity && p.proj_alt) {
         p.proj = p.proj_alt;
     }
+    p.parity = parity;
     field_body<WAKE, CSR>(p, 0, flds);
 }
 
@@ -349,7 +350,7 @@ static void step_free(vfps_step* s)
     hipFree(s->wake_g);
     hipFree(s->csr_h);
     hipFree(s->csr_part);
-    hipFree(s->ticket);
+    hipFree(s->csr_valid);
     hipFree(s->csr_power);
     hipFree(s->inst_d);
     hipFree(s->fparams_d);
@@ -376,7 +377,7 @@ static int step_upload_desc(vfps_step* s)
     d.wake_scale = s->wfield ? s->wfield->wakescaling : 0.0f;
     d.csr_h = s->csr_h;
     d.csr_part = s->csr_part;
-    d.ticket = s->ticket;
+    d.csr_valid = s->csr_valid;
     d.csr_power = s->csr_power;
     d.filling = s->filling;
     d.rf_off = s->rfm ? s->rfm->offset : nullptr;
@@ -417,6 +418,7 @@ static int step_upload_desc(vfps_step* s)
     p.ws = t1->ws;
     p.filling = s->filling;
     p.counter = s->counter;
+    p.counter_pingpong = s->fused ? 1u : 0u;
     p.csr_hist = s->csr_hist;
     p.hist_cap = s->hist_cap;
     p.power = s->csr_power;
@@ -516,10 +518,10 @@ static int step_setup_direct(vfps_step* s)
         VFPS_HIP(hipMemcpy(s->csr_h, h.data(), sizeof(double) * nd,
                            hipMemcpyHostToDevice));
     }
-    VFPS_HIP(hipMalloc(&s->csr_part, sizeof(double) * n));
-    VFPS_HIP(hipMemset(s->csr_part, 0, sizeof(double) * n));
-    VFPS_HIP(hipMalloc(&s->ticket, sizeof(uint32_t)));
-    VFPS_HIP(hipMemset(s->ticket, 0, sizeof(uint32_t)));
+    VFPS_HIP(hipMalloc(&s->csr_part, sizeof(double) * 2 * n));
+    VFPS_HIP(hipMemset(s->csr_part, 0, sizeof(double) * 2 * n));
+    VFPS_HIP(hipMalloc(&s->csr_valid, sizeof(uint32_t) * 2));
+    VFPS_HIP(hipMemset(s->csr_valid, 0xFF, sizeof(uint32_t) * 2));
     return VFPS_OK;
 }
 
@@ -552,7 +554,59 @@ int step_retile(vfps_step* s, uint32_t T, uint32_t threads)
         step_upload_desc(s);
         return rc;
     }
+    if ((rc = step_reset_pending(s)) != VFPS_OK) return rc;
     return step_drop_graph(s);
+}
+
+// The CSR power of the last step of a direct-form run (the shares at
+// parity^1), one workgroup per instance; the counter is at parity.
+__global__ void __launch_bounds__(256)
+step_finalize_kernel(const InstDesc* __restrict__ descs, uint32_t parity)
+{
+    __shared__ double red[16];
+    const InstDesc& d = descs[blockIdx.x];
+    if (!d.use_csr || !d.counter) {
+        return;
+    }
+    const uint32_t cnt = d.counter[parity];
+    if (cnt < 1 || d.csr_valid[parity ^ 1u] != cnt) {
+        return;
+    }
+    double a = 0.0;
+    for (uint32_t k = threadIdx.x; k < d.tiles; k += blockDim.x) {
+        a += d.csr_part[(size_t)(parity ^ 1u) * d.n + k];
+    }
+    const double p = block_sum(a, red);
+    if (threadIdx.x == 0) {
+        *d.csr_power = (float)p;
+        if (d.csr_hist && cnt - 1u < d.hist_cap) {
+            d.csr_hist[cnt - 1u] = (float)p;
+        }
+    }
+}
+
+int step_reset_pending(vfps_step* s)
+{
+    hipStream_t st = s->dev->stream;
+    VFPS_HIP(hipStreamSynchronize(st));
+    uint32_t c = 0;
+    VFPS_HIP(hipMemcpy(&c, s->counter + (s->parity & 1u), sizeof(uint32_t),
+                       hipMemcpyDeviceToHost));
+    const uint32_t cc[2] = {c, c};
+    VFPS_HIP(hipMemcpy(s->counter, cc, sizeof(cc), hipMemcpyHostToDevice));
+    if (s->csr_valid) {
+        VFPS_HIP(hipMemset(s->csr_valid, 0xFF, sizeof(uint32_t) * 2));
+    }
+    return VFPS_OK;
+}
+
+int launch_step_finalize(const InstDesc* descs_d, uint32_t ninst,
+                         uint32_t parity, hipStream_t st)
+{
+    hipLaunchKernelGGL(step_finalize_kernel, dim3(ninst), dim3(256), 0, st,
+                       descs_d, parity);
+    VFPS_HIP(hipGetLastError());
+    return VFPS_OK;
 }
 
 }  // namespace vfps_impl
@@ -655,11 +709,11 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
                            sizeof(double2) * s->M, hipMemcpyHostToDevice));
     }
     VFPS_HIP(hipMalloc(&s->filling, sizeof(double) * 2));
-    VFPS_HIP(hipMalloc(&s->counter, sizeof(uint32_t)));
+    VFPS_HIP(hipMalloc(&s->counter, sizeof(uint32_t) * 2));
     VFPS_HIP(hipMalloc(&s->err, sizeof(uint32_t)));
     VFPS_HIP(hipMalloc(&s->csr_power, sizeof(float)));
     VFPS_HIP(hipMemset(s->filling, 0, sizeof(double) * 2));
-    VFPS_HIP(hipMemset(s->counter, 0, sizeof(uint32_t)));
+    VFPS_HIP(hipMemset(s->counter, 0, sizeof(uint32_t) * 2));
     VFPS_HIP(hipMemset(s->err, 0, sizeof(uint32_t)));
     VFPS_HIP(hipMemset(s->csr_power, 0, sizeof(float)));
     VFPS_HIP(hipMalloc(&s->inst_d, sizeof(InstDesc)));
@@ -762,6 +816,10 @@ int vfps_step_run(vfps_step* s, uint32_t nsteps)
         if (s->fused) s->parity ^= 1u;
         left--;
     }
+    if (s->direct && nsteps > 0) {
+        if ((rc = launch_step_finalize(s->inst_d, 1, s->parity, st)) != VFPS_OK)
+            return rc;
+    }
     VFPS_HIP(hipStreamSynchronize(st));
     uint32_t err = 0;
     VFPS_HIP(hipMemcpy(&err, s->err, sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -789,7 +847,10 @@ int vfps_step_record(vfps_step* s, uint32_t capacity, int want_profiles)
                                sizeof(float) * (size_t)capacity * s->desc.n));
         }
     }
-    VFPS_HIP(hipMemset(s->counter, 0, sizeof(uint32_t)));
+    VFPS_HIP(hipMemset(s->counter, 0, sizeof(uint32_t) * 2));
+    if (s->csr_valid) {
+        VFPS_HIP(hipMemset(s->csr_valid, 0xFF, sizeof(uint32_t) * 2));
+    }
     int rc = step_drop_graph(s);
     if (rc != VFPS_OK) return rc;
     return step_upload_desc(s);
@@ -801,7 +862,9 @@ int vfps_step_get_history(vfps_step* s, uint32_t* count, float* csr_power,
     VFPS_CHECK_ARG(s, "vfps_step_get_history: NULL step");
     VFPS_HIP(hipStreamSynchronize(s->dev->stream));
     uint32_t c = 0;
-    VFPS_HIP(hipMemcpy(&c, s->counter, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t at = s->fused ? (s->parity & 1u) : 0u;
+    VFPS_HIP(hipMemcpy(&c, s->counter + at, sizeof(uint32_t),
+                       hipMemcpyDeviceToHost));
     c = std::min(c, s->hist_cap);
     if (count) *count = c;
     if (csr_power && s->csr_hist && c) {
@@ -848,6 +911,10 @@ int vfps_step_set_mesh(vfps_step* s, const float* mesh)
     VFPS_HIP(hipStreamSynchronize(s->dev->stream));
     VFPS_HIP(hipMemcpy(s->t1->data, mesh, sizeof(float) * s->t1->cells(),
                        hipMemcpyHostToDevice));
+    if (s->fused) {
+        int rc0 = step_reset_pending(s);
+        if (rc0 != VFPS_OK) return rc0;
+    }
     s->parity = 0;
     int rc = launch_project_x(s->t1, s->t1->data, s->t1->proj);
     if (rc != VFPS_OK) return rc;

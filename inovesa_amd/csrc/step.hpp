@@ -25,8 +25,9 @@ struct InstDesc {
     float wake_scale;             // wake scaling divided by nmax
     int32_t use_wake, use_csr;
     const double* csr_h;          // direct CSR: h[d + n - 1], |d| < n
-    double* csr_part;             // [tiles] CSR shares of the tiles
-    uint32_t* ticket;             // tiles finished in this step
+    double* csr_part;             // [2][n] CSR shares of the tiles, by parity
+    uint32_t* csr_valid;          // [2] step (1-based) whose shares csr_part
+                                  // holds, by parity (0xFFFFFFFF: none)
     float* csr_power;             // [1] CSR power of the last step
     double* filling;              // [2] FP64 charge integral (+ total)
     const float* rf_off;          // [n] RF kick offsets (scheme 0)
@@ -37,7 +38,9 @@ struct InstDesc {
     float* prof_hist;             // optional per-step profile history
     float* csr_hist;              // optional per-step CSR power history
     uint32_t hist_cap;
-    uint32_t* counter;            // completed steps
+    uint32_t* counter;            // [2] completed steps, read at parity p,
+                                  // written at p^1 by tile 0 (direct form)
+                                  // or by the field kernel
     uint32_t* err;                // set if a tap falls outside the halo
     uint32_t seq_proj;            // 1: x projection in the reference's order
     unsigned long long* tstamp;   // optional timestamps (phases of tile 0,
@@ -73,6 +76,10 @@ void toeplitz_tables(uint32_t n, uint32_t M, const float2* Z,
 int launch_fields(bool wake, bool csr, const FieldParams* params_d,
                   uint32_t ninst, uint32_t Mmax, uint32_t parity,
                   hipStream_t st);
+// close the last step of a direct-form run: its CSR power from the tiles'
+// shares (the next step's tile 0 does this otherwise)
+int launch_step_finalize(const InstDesc* descs_d, uint32_t ninst,
+                         uint32_t parity, hipStream_t st);
 
 // exact halo of a kick along x (drift) with the given offsets
 void kick_halo(const std::vector<float>& off, uint32_t n, int it,
@@ -97,7 +104,7 @@ struct vfps_step {
     double2* csr_tw = nullptr;
     float csr_fdelta = 0;
     double* filling = nullptr;  // [2]
-    uint32_t* counter = nullptr;
+    uint32_t* counter = nullptr;    // [2] (ping-pong in the fused step)
     uint32_t* err = nullptr;
     float* csr_hist = nullptr;
     float* prof_hist = nullptr;
@@ -105,8 +112,8 @@ struct vfps_step {
     // direct wake / CSR forms
     double* wake_g = nullptr;   // [2n-1]
     double* csr_h = nullptr;    // [2n-1]
-    double* csr_part = nullptr; // [tiles]
-    uint32_t* ticket = nullptr;
+    double* csr_part = nullptr; // [2][n]
+    uint32_t* csr_valid = nullptr;  // [2]
     float* csr_power = nullptr;
     vfps_impl::InstDesc* inst_d = nullptr;
     vfps_impl::FieldParams* fparams_d = nullptr;
@@ -124,4 +131,10 @@ struct vfps_step {
 
 namespace vfps_impl {
 int step_retile(vfps_step* s, uint32_t T, uint32_t threads);
+}  // namespace vfps_impl
+
+namespace vfps_impl {
+// forget pending CSR shares and keep the step count at parity 0 (after a
+// retile, a new mesh or a new history)
+int step_reset_pending(vfps_step* s);
 }  // namespace vfps_impl

@@ -19,7 +19,8 @@ __all__ = ["Device", "PhaseSpace", "KickMap", "DriftMap", "RFKickMap",
            "ConstImpedance", "CollimatorImpedance", "makeImpedance",
            "ElectricField", "Step", "derive", "airy", "VfpsError",
            "VfpsInvalid", "InterpolationType", "Axis", "FPType",
-           "DerivationType", "toeplitz_kernels", "Batch"]
+           "DerivationType", "toeplitz_kernels", "Batch", "MLP",
+           "gemm_bf16", "round_bf16"]
 
 
 class InterpolationType:  # inc/SM/SourceMap.hpp:22-27
@@ -696,4 +697,106 @@ class Batch:
         h = getattr(self, "h", None)
         if h:
             lib.vfps_batch_destroy(h)
+            self.h = None
+
+
+def round_bf16(x):
+    """The kernels' BF16 rounding (nearest even), as float32."""
+    x = _f32(x)
+    y = np.empty_like(x)
+    call("vfps_round_bf16", _p(x), _p(y), x.size)
+    return y
+
+
+def gemm_bf16(A, B, dev=None):
+    """A @ B on the matrix cores (operands rounded to BF16, FP32
+    accumulation); A, B 2-D arrays (any strides are taken from numpy)."""
+    dev = dev or Device.default()
+    A = np.asarray(A, np.float32)
+    B = np.asarray(B, np.float32)
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2
+    a = np.ascontiguousarray(A)
+    b = np.ascontiguousarray(B)
+    C_ = np.empty((M, N), np.float32)
+    call("vfps_gemm_bf16", dev.h, M, N, K, _p(a), a.size, K, 1, _p(b),
+         b.size, N, 1, _p(C_))
+    return C_
+
+
+class MLP:
+    """The surrogate network (vfps_mlp_* in include/vfps_gpu.h): dims =
+    (n_in, hidden..., n_out); GELU after every hidden layer; the last output
+    is a logit."""
+
+    def __init__(self, dims, max_batch=256, seed=0, dev=None):
+        self.dev = dev or Device.default()
+        self.dims = tuple(int(d) for d in dims)
+        arr = np.asarray(self.dims, np.uint32)
+        h = C.c_void_p()
+        call("vfps_mlp_create", self.dev.h, _p(arr), len(self.dims),
+             max_batch, seed, C.byref(h))
+        self.h = h
+        n = C.c_size_t()
+        call("vfps_mlp_nparams", self.h, C.byref(n))
+        self.nparams = n.value
+        self.max_batch = max_batch
+
+    def layers(self, flat):
+        """Split a flat parameter vector into [(W, b), ...]."""
+        out, o = [], 0
+        for a, b in zip(self.dims[:-1], self.dims[1:]):
+            W = flat[o:o + a * b].reshape(b, a)
+            o += a * b
+            out.append((W, flat[o:o + b]))
+            o += b
+        return out
+
+    def params(self):
+        p = np.empty(self.nparams, np.float32)
+        call("vfps_mlp_get_params", self.h, _p(p), None, self.nparams)
+        return p
+
+    def grads(self):
+        g = np.empty(self.nparams, np.float32)
+        call("vfps_mlp_get_params", self.h, None, _p(g), self.nparams)
+        return g
+
+    def set_params(self, p):
+        p = _f32(p)
+        call("vfps_mlp_set_params", self.h, _p(p), p.size)
+
+    def forward(self, x):
+        x = _f32(x).reshape(-1, self.dims[0])
+        y = np.empty((x.shape[0], self.dims[-1]), np.float32)
+        call("vfps_mlp_forward", self.h, _p(x), x.shape[0], _p(y))
+        return y
+
+    def loss_grad(self, x, target, label, label_weight=1.0):
+        x = _f32(x).reshape(-1, self.dims[0])
+        t = _f32(target).reshape(x.shape[0], self.dims[-1] - 1)
+        lab = _f32(label).reshape(x.shape[0])
+        loss = C.c_float()
+        call("vfps_mlp_loss_grad", self.h, _p(x), _p(t), _p(lab), x.shape[0],
+             label_weight, C.byref(loss))
+        return loss.value
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        call("vfps_mlp_adam_step", self.h, lr, beta1, beta2, eps)
+
+    def train(self, x, target, label, batch=256, epochs=10, lr=1e-3,
+              label_weight=1.0, seed=0):
+        x = _f32(x).reshape(-1, self.dims[0])
+        t = _f32(target).reshape(x.shape[0], self.dims[-1] - 1)
+        lab = _f32(label).reshape(x.shape[0])
+        losses = np.zeros(epochs, np.float32)
+        call("vfps_mlp_train", self.h, _p(x), _p(t), _p(lab), x.shape[0],
+             batch, epochs, lr, label_weight, seed, _p(losses))
+        return losses
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib.vfps_mlp_destroy(h)
             self.h = None

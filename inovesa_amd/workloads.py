@@ -73,10 +73,22 @@ def kick_drift_config(n=512, steps_per_ts=2000, gap=0.032, current_factor=1.5,
     return desc, d
 
 
-def rotation_config(n=256, steps_per_ts=1000, damping_steps=1000.0,
+def fp_diffusion_number(n, e1, pq_size=12.0):
+    """e1 / dp^2 of the FokkerPlanckMap diffusion stencil
+    (src/SM/FokkerPlanckMap.cpp:38-118) on an n-point momentum axis of
+    pq_size: the explicit update stays bounded only up to 1/2."""
+    dp = pq_size / (n - 1)
+    return float(e1) / (dp * dp)
+
+
+def rotation_config(n=256, steps_per_ts=1000, damping_steps=1.0e4,
                     current_factor=0.5, Ib=None, padding=8.0):
     """Plain VFP, rotation scheme: WakePotentialMap, RotationMap (cubic,
-    clamped, precomputed semantics), FokkerPlanckMap, free-space CSR."""
+    clamped, precomputed semantics), FokkerPlanckMap, free-space CSR.
+    The damping time is damping_steps time steps: 1000 steps gives
+    e1 = 2e-3, whose diffusion number e1/dp^2 exceeds 1/2 from n = 190 on
+    (0.90 at n = 256), where the explicit Fokker-Planck update of the
+    reference diverges; 1e4 steps keeps it at 0.09 (n = 256)."""
     base = derive(grid_size=n, steps_per_ts=steps_per_ts, vacuum_gap=-1.0,
                   padding=padding)
     dt = base["dt"]
@@ -114,15 +126,17 @@ def impedances(d, dev=None, resistive_wall=None):
 
 def batched_scan_config(sizes=(128, 256, 384, 512), per_size=16, seed=3,
                         current_range=(0.3, 3.0), gap_range=(0.016, 0.064),
-                        steps_per_ts=1000, damping_steps=1000.0):
+                        steps_per_ts=1000, damping_steps=1.0e4):
     """Independent instances of uneven mesh sizes for a threshold scan:
     per_size instances of each size (sizes interleaved), currents
     log-spaced over current_range times each instance's scaling-law
     threshold, parallel-plates gaps drawn uniformly from gap_range (m) with
     the given seed, and the rotation scheme (cubic clamped RotationMap,
     FokkerPlanckMap with damping time damping_steps steps, wake kick) with
-    each instance's own parallel-plates CSR impedance.  Returns a list of
-    (desc, derived, init) per instance."""
+    each instance's own parallel-plates CSR impedance (the default damping
+    keeps the diffusion number below 1/2 up to n = 512; see
+    rotation_config).  Returns a list of (desc, derived, init) per
+    instance."""
     k = len(sizes) * per_size
     rng = np.random.Generator(np.random.Philox(seed))
     gaps = rng.uniform(gap_range[0], gap_range[1], k)

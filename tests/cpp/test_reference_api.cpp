@@ -179,9 +179,12 @@ static void electric_field_cases(oclhptr_t oclh)
     for (meshindex_t i = 0; i < n; i++) {
         CHECK_SMALL(wp[i] - proj[i] * s * spacing, 1e-6);
     }
-    // the forward wake of a local change does not reach earlier bins
+    // the forward wake of a local change does not reach earlier bins (the
+    // scaling of a 1 mA bunch, as in the reference test)
     auto zc = std::make_shared<FreeSpaceCSR>(spacing, 1e6f, 1e12f, oclh);
-    ElectricField fc(ps, zc, {0}, spacing, oclh, 1e6, 0.1f, 1.0f);
+    const float ws = vfps_field_wakescaling(1e-3, 1e-3, 3.0, ps->getDelta(1),
+                                            1e3, 1e9);
+    ElectricField fc(ps, zc, {0}, spacing, oclh, 1e6, 0.1f, ws);
     const std::vector<float> w0(fc.wakePotential(), fc.wakePotential() + n);
     std::vector<float> p2 = proj;
     const meshindex_t c = n / 2;

@@ -25,7 +25,7 @@ def _setup(V, scheme, n, **kw):
 
 
 CASES = [(0, 512, {}), (1, 256, {}), (0, 256, {"steps_per_ts": 500}),
-         (1, 128, {"steps_per_ts": 300})]
+         (1, 128, {"steps_per_ts": 300, "damping_steps": 1000.0})]
 
 
 @pytest.mark.parametrize("method", [1, 2])
