@@ -11,9 +11,11 @@ Prints one JSON line.  value = mesh-cell updates per second
 (n^2 * steps / wall time of the timed steps, state resident on the GPU).
 roofline: the step kernel (the whole step in one launch; 8 bytes of mesh
 traffic per cell) timed with HIP events on its stream.  fft_wake: the same
-workload with the wake and CSR power from the FFT field kernel.  cpu_baseline: the oracle (the FP32 C
-restatement of the reference CPU path) on one host core, on a bounded
-sample of the same workload.
+workload with the wake and CSR power from the FFT field kernel.  batched:
+64 independent meshes of 128^2-512^2 in one launch per step.  scan: the
+threshold scan with and without the trained network.  cpu_baseline: the
+oracle (the FP32 C restatement of the reference CPU path) on one host
+core, on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -51,6 +53,63 @@ def cpu_baseline(desc, zw, zc, init, budget_s=15.0):
                       f"state, one thread ({dt:.1f} s)"}
 
 
+def batched_throughput(steps, log):
+    """Independent meshes of uneven sizes (16 each of 128^2, 256^2, 384^2,
+    512^2; currents log-spaced over 0.3-3x threshold; gaps 16-64 mm;
+    rotation scheme; each its own impedance) advanced together after the
+    autotuner picked the tiles: whole-batch cell updates per second."""
+    import numpy as np
+    from inovesa_amd import Batch
+    from inovesa_amd.workloads import batched_scan_config, impedances
+    cfg = batched_scan_config()
+    zs = [impedances(d) for _, d, _ in cfg]
+    b = Batch([c[0] for c in cfg], [c[2] for c in cfg], [z[0] for z in zs],
+              [z[1] for z in zs])
+    table = b.autotune(steps=10)
+    cells = b.info()["cells"]
+    b.record(steps + 128)
+    b.run(128)
+    b.dev.sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.dev.sync()
+    dt = time.perf_counter() - t0
+    chosen = {r["n"]: (r["T"], r["threads"], round(r["us_per_step"], 2))
+              for r in table if r["chosen"]}
+    log(f"batched: {len(cfg)} instances, {cells} cells, {steps} steps in "
+        f"{dt:.2f} s")
+    return {"value": cells * steps / dt, "unit": "cell-updates/s",
+            "instances": len(cfg), "cells_per_step": int(cells),
+            "steps": steps, "ms_per_step": dt * 1e3 / steps,
+            "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
+            "tiles_chosen": {str(k): v for k, v in chosen.items()}}
+
+
+def scan_ablation(targets, log):
+    """Threshold scans of `targets` (gap, damping) pairs: the plain dense
+    scan and the scan steered by the trained network."""
+    from inovesa_amd.pipeline import compare, scan_targets
+    from inovesa_amd.scan import ScanSetup
+    from inovesa_amd.surrogate import Surrogate
+    res = compare(Surrogate.load(), scan_targets(targets), ScanSetup())
+    log(f"scan: dense {res['dense_gpu_s']:.2f} s, surrogate "
+        f"{res['surrogate_gpu_s']:.2f} s, ratio {res['gpu_time_ratio']:.3f}")
+    return {k: v for k, v in res.items() if k != "rows"}
+
+
+def copy_calibration(mib):
+    """A plain device copy of `mib` MiB (read + write), for scaling the HBM
+    counters of a profiler run to bytes."""
+    import torch
+    a = torch.ones(mib * 262144, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_mib": mib, "bytes_read": mib * 1048576,
+                      "bytes_written": mib * 1048576}), flush=True)
+
+
 def load_traffic(name):
     p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(p):
@@ -67,7 +126,17 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--scan-targets", type=int, default=48)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB and exit")
     args = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
               "not supported (independent instances shard across GPUs by "
@@ -136,6 +205,10 @@ def main():
                       "tile_rows": cfg["tile_rows"],
                       "threads": cfg["threads"]},
            "roofline": roofline, "fft_wake": fft_line}
+    if not args.no_batch:
+        out["batched"] = batched_throughput(args.batch_steps, log)
+    if not args.no_scan:
+        out["scan"] = scan_ablation(args.scan_targets, log)
     if not args.no_cpu:
         cb = cpu_baseline(desc, zw, zc, init, args.cpu_budget)
         cb["ratio_gpu_over_cpu"] = value / cb["value"]

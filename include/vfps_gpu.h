@@ -354,9 +354,11 @@ VFPS_API int vfps_step_get_history(vfps_step* s, uint32_t* count,
 VFPS_API int vfps_step_get_mesh(vfps_step* s, float* mesh, float* proj,
                                 float* wake);
 VFPS_API int vfps_step_set_mesh(vfps_step* s, const float* mesh);
-/* time the last run(): average duration of each kernel of the step, from
- * HIP events on the step's stream (milliseconds); names are
- * "wake", "maps" */
+/* Advance nsteps and time the kernels with HIP events on the step's stream
+ * (milliseconds per step): the direct form launches its one kernel nsteps
+ * times back to back between two events (ms_maps; ms_wake = 0); the FFT
+ * form brackets the field kernel (ms_wake) and the step kernel (ms_maps) of
+ * each step. */
 VFPS_API int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
                               float* ms_maps, float* ms_step);
 /* one fused step with device timestamps at the phase boundaries of the
@@ -391,6 +393,12 @@ VFPS_API int vfps_batch_get_history(vfps_batch* b, uint32_t inst,
                                     float* profiles);
 VFPS_API int vfps_batch_get_mesh(vfps_batch* b, uint32_t inst, float* mesh,
                                  float* proj, float* wake);
+/* the current x projection of every instance, concatenated (sum of n) */
+VFPS_API int vfps_batch_get_projections(vfps_batch* b, float* out);
+/* recorded CSR power of steps [first, first+count) of every instance,
+ * [ninst][count] */
+VFPS_API int vfps_batch_get_csr_histories(vfps_batch* b, uint32_t first,
+                                          uint32_t count, float* out);
 /* instances, launch groups and mesh cells per step */
 VFPS_API int vfps_batch_info(vfps_batch* b, uint32_t* ninst,
                              uint32_t* ngroups, uint64_t* cells);
@@ -400,7 +408,7 @@ VFPS_API int vfps_batch_config(vfps_batch* b, uint32_t inst, uint32_t* out,
 /* average time of one batch step (all launches), HIP events on the device
  * stream; advances the state by nsteps */
 VFPS_API int vfps_batch_timing(vfps_batch* b, uint32_t nsteps, float* ms_step);
-/* Autotuner: per mesh size, time every tile height T in {1,2,4,8} and
+/* Autotuner: per mesh size, time every tile height T in {1,2,4,8,16} and
  * workgroup size in {128,256,512} that fits (steps steps each, on that
  * size's instances alone) and keep the fastest.  The instances' state is
  * restored afterwards; results do not depend on the choice.  table (cap

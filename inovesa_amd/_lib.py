@@ -165,6 +165,8 @@ _SIGS = {
     "vfps_batch_get_history": (_int, [_vp, _u32, _vp, _vp, _vp]),
     "vfps_batch_get_mesh": (_int, [_vp, _u32, _vp, _vp, _vp]),
     "vfps_batch_info": (_int, [_vp, _vp, _vp, _vp]),
+    "vfps_batch_get_projections": (_int, [_vp, _vp]),
+    "vfps_batch_get_csr_histories": (_int, [_vp, _u32, _u32, _vp]),
     "vfps_batch_config": (_int, [_vp, _u32, _vp, _u32]),
     "vfps_batch_timing": (_int, [_vp, _u32, _vp]),
     "vfps_batch_autotune": (_int, [_vp, _u32, _vp, _u32, _vp]),

@@ -343,6 +343,43 @@ int vfps_batch_get_mesh(vfps_batch* b, uint32_t inst, float* mesh,
     return vfps_step_get_mesh(b->members[inst], mesh, proj, wake);
 }
 
+int vfps_batch_get_projections(vfps_batch* b, float* out)
+{
+    VFPS_CHECK_ARG(b && out, "vfps_batch_get_projections: NULL argument");
+    hipStream_t st = b->dev->stream;
+    VFPS_HIP(hipStreamSynchronize(st));
+    sync_parity(b);
+    size_t o = 0;
+    for (auto* s : b->members) {
+        const vfps_mesh* cur = (s->parity & 1u) ? s->t2 : s->t1;
+        VFPS_HIP(hipMemcpyAsync(out + o, cur->proj, sizeof(float) * s->desc.n,
+                                hipMemcpyDeviceToHost, st));
+        o += s->desc.n;
+    }
+    VFPS_HIP(hipStreamSynchronize(st));
+    return VFPS_OK;
+}
+
+int vfps_batch_get_csr_histories(vfps_batch* b, uint32_t first, uint32_t count,
+                                 float* out)
+{
+    VFPS_CHECK_ARG(b && out, "vfps_batch_get_csr_histories: NULL argument");
+    hipStream_t st = b->dev->stream;
+    VFPS_HIP(hipStreamSynchronize(st));
+    size_t o = 0;
+    for (auto* s : b->members) {
+        VFPS_CHECK_ARG(s->csr_hist && first + count <= s->hist_cap,
+                       "vfps_batch_get_csr_histories: outside the recorded "
+                       "capacity");
+        VFPS_HIP(hipMemcpyAsync(out + o, s->csr_hist + first,
+                                sizeof(float) * count, hipMemcpyDeviceToHost,
+                                st));
+        o += count;
+    }
+    VFPS_HIP(hipStreamSynchronize(st));
+    return VFPS_OK;
+}
+
 int vfps_batch_info(vfps_batch* b, uint32_t* ninst, uint32_t* ngroups,
                     uint64_t* cells)
 {
@@ -437,7 +474,7 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
         uint32_t bestThr = b->members[cl.second[0]]->threads;
         const uint32_t first_row = rows;
         for (uint32_t thr : {128u, 256u, 512u}) {
-            for (uint32_t T : {1u, 2u, 4u, 8u}) {
+            for (uint32_t T : {1u, 2u, 4u, 8u, 16u}) {
                 if (ceil_div(n, thr) > (uint32_t)STEP_MAX_COLS || T > n) continue;
                 bool ok = true;
                 for (uint32_t i : cl.second) {

@@ -17,8 +17,9 @@
 //      products mesh*weight of the x projection;
 //   G  the x projection of its rows (PhaseSpace::updateXProjection), summed
 //      in std::inner_product order;
-//   H  (direct wake) the workgroup that finishes last sums the CSR shares in
-//      a fixed order, computes the FP64 charge integral and counts the step.
+//   H  (direct wake) every tile stores its CSR share; tile 0 counts the
+//      step, computes the FP64 charge integral and sums the previous step's
+//      shares in a fixed order (the launch boundary orders them).
 // The mesh makes one trip HBM -> LDS -> HBM per step.  The state buffers
 // and the x projections ping-pong (`parity`), so a step reads only what
 // the previous launch wrote.
@@ -44,7 +45,9 @@ namespace vfps_impl {
 // come first so they stay 8-byte aligned)
 struct StepLds {
     uint32_t gwin, hwin;          // FP64 kernel windows (word offsets)
+    uint32_t g0win;               // FP64 kernel window of row 0 (g[0..n))
     uint32_t pprev;               // previous x projection
+    uint32_t row0;                // input row 0 (scheme 1 out-of-range taps)
     uint32_t bufA, bufB;          // row buffers, `stride` rows each
     uint32_t tile_out;            // [T][n + 4] projection products
     uint32_t offw, offr;          // per-row kick offsets
@@ -54,6 +57,14 @@ struct StepLds {
 };
 
 __host__ __device__ inline uint32_t step_round4(uint32_t v) { return (v + 3u) & ~3u; }
+
+// The direct form's dot products (one per wake row, CSR row and, for the
+// rotation, row 0) are each summed by one wavefront in a fixed order that
+// depends on neither the tile height nor the workgroup size: lane l takes
+// x' = l, l + 64, ... in increasing order, then a butterfly over the lanes.
+// LDS words of one row's kick taps
+constexpr uint32_t STEP_TAP_WORDS = 8;
+
 
 __host__ __device__ inline StepLds step_lds(uint32_t n, uint32_t T, uint32_t stride,
                                             bool direct, uint32_t it)
@@ -67,8 +78,12 @@ __host__ __device__ inline StepLds step_lds(uint32_t n, uint32_t T, uint32_t str
         w += 2 * (n + T);
         L.csr_rows = w;
         w += 2 * T;
+        L.g0win = w;
+        w += 2 * n;
         w = step_round4(w);
         L.pprev = w;
+        w += step_round4(n);
+        L.row0 = w;
         w += step_round4(n);
     }
     L.bufA = w;
@@ -82,9 +97,9 @@ __host__ __device__ inline StepLds step_lds(uint32_t n, uint32_t T, uint32_t str
     L.offr = w;
     w += step_round4(stride);
     L.tapw = w;
-    w += step_round4(2 * it * stride);
+    w += STEP_TAP_WORDS * stride;
     L.tapr = w;
-    w += step_round4(2 * it * stride);
+    w += STEP_TAP_WORDS * stride;
     L.words = w;
     return L;
 }
@@ -114,22 +129,34 @@ constexpr uint32_t STEP_NO_TILE = 0xFFFFFFFFu;
         }                                                                     \
     } while (0)
 
-// taps of one kick row, stored as IT shifts (index - n/2) then IT weights
+// Taps of one kick row (KickMap::updateSM, src/SM/KickMap.cpp:349-386):
+// STEP_TAP_WORDS words per row, the IT weights (zero-padded to 4) then the
+// shift of tap 0 relative to the output sample.  The taps of a row are
+// consecutive, index jd-(IT-1)/2 .. jd+IT/2; a tap the reference sends to
+// index n/2 (out of range) has weight zero, so giving it the consecutive
+// index instead adds the same +0 to the sum.
+
 template <int IT>
 __device__ __forceinline__ void store_row_taps(float off, uint32_t n, float* taps)
 {
     uint32_t idx[IT];
     float w[IT];
     kick_taps<IT>(off, n, idx, w);
+    const float poffs = (float)(n / 2) + off;
+    float qi;
+    vf_modf(poffs, &qi);
+    const uint32_t jd = vf_f2u(qi);
+    const int32_t base = (int32_t)jd - (IT - 1) / 2 - (int32_t)(n / 2);
 #pragma unroll
-    for (int j = 0; j < IT; j++) {
-        taps[j] = __int_as_float((int32_t)idx[j] - (int32_t)(n / 2));
-        taps[IT + j] = w[j];
+    for (int j = 0; j < 4; j++) {
+        taps[j] = j < IT ? w[j] : 0.0f;
     }
+    taps[4] = __int_as_float(base);
 }
 
 // KickMap::apply along y (src/SM/KickMap.cpp:244-267) on local rows
-// [r0, r1) of a row buffer, every (row, y) pair spread over the workgroup
+// [r0, r1) of a row buffer: each thread makes four consecutive outputs of
+// a row from IT+3 consecutive inputs, taps in order (n % 4 == 0).
 template <int IT>
 __device__ __forceinline__ void kick_rows(const float* __restrict__ src,
                                           float* __restrict__ dst, uint32_t n,
@@ -140,43 +167,42 @@ __device__ __forceinline__ void kick_rows(const float* __restrict__ src,
         return;
     }
     const uint32_t bs = blockDim.x;
-    const uint32_t drow = bs / n, dy = bs % n;
-    uint32_t row = (uint32_t)r0 + threadIdx.x / n;
-    uint32_t y = threadIdx.x % n;
-#pragma unroll 2
-    for (; row < (uint32_t)r1;) {
-        const float* tp = taps + row * (2 * IT);
+    const uint32_t ng = n >> 2;
+    const uint32_t drow = bs / ng, dg = bs % ng;
+    uint32_t row = (uint32_t)r0 + threadIdx.x / ng;
+    uint32_t g = threadIdx.x % ng;
+    while (row < (uint32_t)r1) {
+        const float* tp = taps + row * STEP_TAP_WORDS;
+        const vec4f w4 = *reinterpret_cast<const vec4f*>(tp);
+        const int32_t base = __float_as_int(tp[4]);
         const float* s = src + (size_t)row * n;
-        float v = 0.0f;
+        const int32_t y0 = (int32_t)(4 * g);
+        float v[IT + 3];
 #pragma unroll
-        for (int j = 0; j < IT; j++) {
-            const uint32_t ys = (uint32_t)((int32_t)y + __float_as_int(tp[j]));
-            if (ys < n) {
-                v = v + s[ys] * tp[IT + j];
-            }
+        for (int k = 0; k < IT + 3; k++) {
+            const uint32_t ys = (uint32_t)(y0 + base + k);
+            v[k] = ys < n ? s[ys] : 0.0f;
         }
-        dst[(size_t)row * n + y] = v;
-        y += dy;
+        // an input out of range is skipped by the reference; here it is a
+        // zero, whose product adds +0 or -0 to the sum: the same result
+        vec4f o;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int j = 0; j < IT; j++) {
+                acc = acc + v[e + j] * w4[j];
+            }
+            o[e] = acc;
+        }
+        *reinterpret_cast<vec4f*>(dst + (size_t)row * n + 4 * g) = o;
+        g += dg;
         row += drow;
-        if (y >= n) {
-            y -= n;
+        if (g >= ng) {
+            g -= ng;
             row++;
         }
     }
-}
-
-// Toeplitz dot product of the previous profile with a kernel window:
-// sum_x' win[base + n - 1 - x'] p[x'] in FP64, over one wavefront
-__device__ __forceinline__ double toeplitz_dot(const double* __restrict__ win,
-                                               uint32_t base,
-                                               const float* __restrict__ p,
-                                               uint32_t n, uint32_t lane)
-{
-    double acc = 0.0;
-    for (uint32_t x = lane; x < n; x += 64) {
-        acc += win[base + n - 1 - x] * (double)p[x];
-    }
-    return wave_sum(acc);
 }
 
 template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP>
@@ -188,7 +214,7 @@ step_kernel(const InstDesc* __restrict__ descs,
     __shared__ double red[16];
     __shared__ float s00_sh;
     __shared__ uint32_t slot_sh;
-    __shared__ uint32_t last_sh;
+    __shared__ uint32_t cnt_sh;
     const unsigned long long t_entry = wall_clock64();
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
     // so give each XCD a contiguous run of tiles (neighbouring tiles share
@@ -255,9 +281,10 @@ step_kernel(const InstDesc* __restrict__ descs,
     const bool csr = DIRECT && d.use_csr != 0;
 
     // ---- A: every global read -------------------------------------------
-    uint32_t cnt = 0;
-    if (d.counter && tid == 0) {
-        cnt = *d.counter;
+    // steps completed before this one (read by one thread, shared with the
+    // workgroup after the barrier that ends phase A)
+    if (tid == 0) {
+        cnt_sh = d.counter ? d.counter[parity] : 0u;
     }
     //  - previous profile and the kernel windows (direct wake)
     //    wake rows [rbase+lr0, rbase+lr1) need g[r - x' + n - 1]:
@@ -281,6 +308,16 @@ step_kernel(const InstDesc* __restrict__ descs,
             }
         }
     }
+    //  - row 0 of the input, for the out-of-range taps of the rotation in
+    //    tiles that do not hold it
+    if (DIRECT && SCHEME == 1 && rbase > 0 && wake) {
+        float* row0 = lds + L.row0;
+        double* g0win = reinterpret_cast<double*>(lds + L.g0win);
+        for (uint32_t y = tid; y < n; y += bs) {
+            row0[y] = in[y];
+            g0win[y] = wake_g[y];
+        }
+    }
     //  - per-row kick offsets; the RF taps (and FFT-wake taps) right away
     if ((int)tid >= lr0 && (int)tid < lr1) {
         const int r = rbase + (int)tid;
@@ -293,14 +330,18 @@ step_kernel(const InstDesc* __restrict__ descs,
     }
     //  - per-column tables in registers
     const int ncols = (int)((n + bs - 1) / bs);
-    float col_drift[STEP_MAX_COLS], col_ws[STEP_MAX_COLS];
+    float col_ws[STEP_MAX_COLS];
+    uint32_t col_didx[STEP_MAX_COLS][IT];   // drift taps of the column
+    float col_dw[STEP_MAX_COLS][IT];
     uint32_t col_fi[STEP_MAX_COLS][IPFP > 0 ? IPFP : 1];
     float col_fw[STEP_MAX_COLS][IPFP > 0 ? IPFP : 1];
 #pragma unroll
     for (int c = 0; c < STEP_MAX_COLS; c++) {
         const uint32_t y = tid + (uint32_t)c * bs;
         if (c < ncols && y < n) {
-            if (SCHEME == 0) col_drift[c] = drift_off[y];
+            if (SCHEME == 0) {
+                kick_taps<IT>(drift_off[y], n, col_didx[c], col_dw[c]);
+            }
             col_ws[c] = ws_g[y];
 #pragma unroll
             for (int k = 0; k < IPFP; k++) {
@@ -340,13 +381,14 @@ step_kernel(const InstDesc* __restrict__ descs,
     }
     if ((int)tid >= lr0 && (int)tid < lr1) {
         if (SCHEME == 0) {
-            store_row_taps<IT>(offr[tid], n, tapr + tid * (2 * IT));
+            store_row_taps<IT>(offr[tid], n, tapr + tid * STEP_TAP_WORDS);
         }
         if (!DIRECT && wake) {
-            store_row_taps<IT>(offw[tid], n, tapw + tid * (2 * IT));
+            store_row_taps<IT>(offw[tid], n, tapw + tid * STEP_TAP_WORDS);
         }
     }
     __syncthreads();
+    const uint32_t cnt = cnt_sh;
     VFPS_STAMP(1);
 
     // ---- B: direct wake offsets and the CSR share --------------------------
@@ -357,47 +399,54 @@ step_kernel(const InstDesc* __restrict__ descs,
     if (DIRECT && (wake || csr)) {
         const int nwk = wake ? lr1 - lr0 : 0;
         const int ncsr = csr ? rows_out : 0;
-        const int njobs = nwk + ncsr + ((need_s00 && wake) ? 1 : 0);
-        for (int job = (int)wave; job < njobs; job += (int)nw) {
+        const int J = nwk + ncsr + ((need_s00 && wake) ? 1 : 0);
+        for (int job = (int)wave; job < J; job += (int)nw) {
+            const double* win = gwin;
+            uint32_t base = 0;
             if (job < nwk) {
-                const double a = toeplitz_dot(gwin, (uint32_t)job, pprev, n, lane);
-                if (lane == 0) {
-                    const int row = lr0 + job;
-                    const float off = d.wake_scale * (float)a;
-                    offw[row] = off;
-                    store_row_taps<IT>(off, n, tapw + row * (2 * IT));
-                    if (row >= hlo && row < hlo + rows_out && d.wake_out) {
-                        d.wake_out[rbase + row] = off;
-                    }
+                base = (uint32_t)job;
+            } else if (job < nwk + ncsr) {
+                win = hwin;
+                base = (uint32_t)(job - nwk);
+            } else {
+                win = reinterpret_cast<const double*>(lds + L.g0win);
+            }
+            double acc = 0.0;
+#pragma unroll 4
+            for (uint32_t x = lane; x < n; x += 64) {
+                acc += win[base + n - 1 - x] * (double)pprev[x];
+            }
+            acc = wave_sum(acc);
+            if (lane != 0) {
+                continue;
+            }
+            if (job < nwk) {
+                const int row = lr0 + job;
+                const float off = d.wake_scale * (float)acc;
+                offw[row] = off;
+                store_row_taps<IT>(off, n, tapw + row * STEP_TAP_WORDS);
+                if (row >= hlo && row < hlo + rows_out && d.wake_out) {
+                    d.wake_out[rbase + row] = off;
                 }
             } else if (job < nwk + ncsr) {
                 const int o = job - nwk;
-                const double u = toeplitz_dot(hwin, (uint32_t)o, pprev, n, lane);
-                if (lane == 0) {
-                    csr_rows[o] = (double)pprev[x0 + o] * u;
-                }
+                csr_rows[o] = (double)pprev[x0 + o] * acc;
             } else {
-                // wake offset of row 0 from the kernel in global memory
-                double acc = 0.0;
-                for (uint32_t x = lane; x < n; x += 64) {
-                    acc += wake_g[n - 1 - x] * (double)pprev[x];
-                }
-                acc = wave_sum(acc);
-                if (lane == 0) {
-                    const float off = d.wake_scale * (float)acc;
-                    uint32_t idx[IT];
-                    float w[IT];
-                    kick_taps<IT>(off, n, idx, w);
-                    float v = 0.0f;
+                // kick row 0 by its wake offset; s00 is its sample y = 0
+                const float* row0 = lds + L.row0;
+                const float off = d.wake_scale * (float)acc;
+                uint32_t idx[IT];
+                float w[IT];
+                kick_taps<IT>(off, n, idx, w);
+                float v = 0.0f;
 #pragma unroll
-                    for (int j = 0; j < IT; j++) {
-                        const uint32_t ys = (uint32_t)((int32_t)idx[j] - half);
-                        if (ys < n) {
-                            v = v + in[ys] * w[j];
-                        }
+                for (int t = 0; t < IT; t++) {
+                    const uint32_t ys = (uint32_t)((int32_t)idx[t] - half);
+                    if (ys < n) {
+                        v = v + row0[ys] * w[t];
                     }
-                    s00_sh = v;
                 }
+                s00_sh = v;
             }
         }
         __syncthreads();
@@ -435,9 +484,8 @@ step_kernel(const InstDesc* __restrict__ descs,
             if (c >= ncols || y >= n) {
                 continue;
             }
-            uint32_t idx[IT];
-            float w[IT];
-            kick_taps<IT>(col_drift[c], n, idx, w);
+            const uint32_t* idx = col_didx[c];
+            const float* w = col_dw[c];
             for (int orow = 0; orow < rows_out; orow++) {
                 const uint32_t x = (uint32_t)(x0 + orow);
                 float v = 0.0f;
@@ -537,14 +585,10 @@ step_kernel(const InstDesc* __restrict__ descs,
         }
     }
     if (tid == 0) {
-        // history slot of this step (the counter before the step in the
-        // direct form; the field kernel has already counted it otherwise)
+        // history slot of this step: the steps completed before it
         uint32_t slot = 0xFFFFFFFFu;
-        if (d.prof_hist && d.counter) {
-            const uint32_t c = DIRECT ? cnt : cnt - 1u;
-            if ((DIRECT || cnt >= 1) && c < d.hist_cap) {
-                slot = c;
-            }
+        if (d.prof_hist && d.counter && cnt < d.hist_cap) {
+            slot = cnt;
         }
         slot_sh = slot;
     }
@@ -559,13 +603,19 @@ step_kernel(const InstDesc* __restrict__ descs,
             const float* r = tile_out + (size_t)tid * tstride;
             float acc = 0.0f;
             uint32_t y = 0;
-            if ((n & 3u) == 0u) {
-                const float4* r4 = reinterpret_cast<const float4*>(r);
-                for (; y + 16 <= n; y += 16) {
-                    float4 q[4];
+            if ((n & 15u) == 0u) {
+                // the next 16 values load while the current 16 are added
+                const vec4f* r4 = reinterpret_cast<const vec4f*>(r);
+                vec4f q[4], nq[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    q[k] = r4[k];
+                }
+                for (; y < n; y += 16) {
+                    const uint32_t yn = y + 16 < n ? y + 16 : y;
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        q[k] = r4[(y >> 2) + k];
+                        nq[k] = r4[(yn >> 2) + k];
                     }
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
@@ -573,6 +623,10 @@ step_kernel(const InstDesc* __restrict__ descs,
                         acc = acc + q[k].y;
                         acc = acc + q[k].z;
                         acc = acc + q[k].w;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        q[k] = nq[k];
                     }
                 }
             }
@@ -603,40 +657,35 @@ step_kernel(const InstDesc* __restrict__ descs,
         }
     }
     VFPS_STAMP(7);
-    // ---- H: the last tile closes the step (direct form) ----------------------
+    // ---- H: close the step (direct form) --------------------------------------
+    // Every tile leaves its CSR share; tile 0 counts the step, integrates
+    // the profile that entered it and sums the previous step's shares
+    // (written by the previous launch, so no workgroup waits for another).
     if (DIRECT) {
-        if (tid == 0) {
-            if (csr) {
-                d.csr_part[tile] = csr_q;
-            }
-            __threadfence();
-            const uint32_t t = atomicAdd(d.ticket, 1u);
-            last_sh = (t == d.tiles - 1u) ? 1u : 0u;
+        if (csr && tid == 0) {
+            d.csr_part[(size_t)parity * n + tile] = csr_q;
         }
-        __syncthreads();
-        if (last_sh) {
-            __threadfence();
-            // CSR power: shares summed in tile order
-            double p = 0.0;
-            if (csr) {
-                double a = 0.0;
-                for (uint32_t k = tid; k < d.tiles; k += bs) {
-                    a += d.csr_part[k];
-                }
-                p = block_sum(a, red);
-            }
-            // FP64 charge integral of the profile that entered the step
-            // (src/PS/PhaseSpace.cpp:200-239)
+        if (tile == 0) {
             double f = 0.0;
             for (uint32_t x = tid; x < n; x += bs) {
                 f += (double)pprev[x] * (double)ws_g[x];
             }
             f = block_sum(f, red);
+            double p = 0.0;
+            // uniform over the workgroup: the sum below has barriers
+            const bool prev = csr && cnt >= 1 && d.csr_valid[parity ^ 1u] == cnt;
+            if (prev) {
+                double a = 0.0;
+                for (uint32_t k = tid; k < d.tiles; k += bs) {
+                    a += d.csr_part[(size_t)(parity ^ 1u) * n + k];
+                }
+                p = block_sum(a, red);
+            }
             if (tid == 0) {
-                if (csr) {
+                if (prev) {
                     *d.csr_power = (float)p;
-                    if (d.csr_hist && cnt < d.hist_cap) {
-                        d.csr_hist[cnt] = (float)p;
+                    if (d.csr_hist && cnt - 1u < d.hist_cap) {
+                        d.csr_hist[cnt - 1u] = (float)p;
                     }
                 }
                 if (d.filling) {
@@ -644,9 +693,9 @@ step_kernel(const InstDesc* __restrict__ descs,
                     d.filling[1] = f;
                 }
                 if (d.counter) {
-                    *d.counter = cnt + 1u;
+                    d.counter[parity ^ 1u] = cnt + 1u;
                 }
-                *d.ticket = 0u;
+                d.csr_valid[parity] = cnt + 1u;
             }
         }
     }

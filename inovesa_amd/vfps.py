@@ -664,6 +664,18 @@ class Batch:
         call("vfps_batch_get_mesh", self.h, i, _p(m), _p(p), _p(w))
         return m, p, w
 
+    def projections(self):
+        """Current x projection of every instance (list of arrays)."""
+        out = np.empty(sum(self.n), np.float32)
+        call("vfps_batch_get_projections", self.h, _p(out))
+        return np.split(out, np.cumsum(self.n)[:-1])
+
+    def csr_histories(self, first, count):
+        """Recorded CSR power of steps [first, first+count), [ninst][count]."""
+        out = np.empty((len(self.n), count), np.float32)
+        call("vfps_batch_get_csr_histories", self.h, first, count, _p(out))
+        return out
+
     def info(self):
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint64()
         call("vfps_batch_info", self.h, C.byref(a), C.byref(b), C.byref(c))
