@@ -1,10 +1,11 @@
 """Train the threshold surrogate on the batched solver and compare the
 surrogate-steered scan with the plain dense scan.
 
-    python -m inovesa_amd.pipeline --runs 4096 --out DIR
+    python -m inovesa_amd.pipeline --runs 8192 --focus 0.6 --out DIR
 
 Steps (all on the GPU through the library):
-  1. `runs` VFP runs at random (current, gap, damping) points, batched;
+  1. `runs` VFP runs at random (current, gap, damping) points, batched,
+     a fraction `focus` of them at currents near the thresholds;
      per run the 64-bin log CSR spectrum and the detector's verdict;
   2. the network trained on them with the library's forward, backward and
      Adam kernels;
@@ -42,6 +43,8 @@ def compare(model, targets, setup, rel_tol=0.02):
     for a, b in zip(dres, sres):
         rows.append(dict(gap=a.gap, damping=a.damping,
                          dense=a.threshold, dense_width=a.width,
+                         predicted=b.predicted,
+                         prediction_error=b.predicted / a.threshold - 1.0,
                          surrogate=b.threshold, surrogate_width=b.width,
                          confirmed=b.confirmed, dense_runs=a.runs,
                          surrogate_runs=b.runs,
@@ -53,12 +56,15 @@ def compare(model, targets, setup, rel_tol=0.02):
         dense_runs=drun.runs, surrogate_runs=srun.runs,
         gpu_time_ratio=srun.gpu_seconds / max(drun.gpu_seconds, 1e-12),
         max_rel_diff=max(r["rel_diff"] for r in rows),
+        max_prediction_error=max(abs(r["prediction_error"]) for r in rows),
         confirmed=sum(r["confirmed"] for r in rows), rows=rows)
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--runs", type=int, default=4096)
+    ap.add_argument("--runs", type=int, default=8192)
+    ap.add_argument("--focus", type=float, default=0.6,
+                    help="fraction of the runs near the threshold currents")
     ap.add_argument("--chunk", type=int, default=1024)
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--targets", type=int, default=8)
@@ -74,7 +80,7 @@ def main(argv=None):
         model = Surrogate.load(args.model)
     else:
         pts, spec, lab, fl, gpu_s = generate(args.runs, setup, chunk=args.chunk,
-                                             log=log)
+                                             log=log, focus=args.focus)
         np.savez_compressed(os.path.join(args.out, "training_data.npz"),
                             points=pts, spectra=spec, labels=lab, fluct=fl)
         log(f"data: {len(lab)} runs, {int(lab.sum())} unstable, "
@@ -93,7 +99,7 @@ def main(argv=None):
             f"{losses[0]:.4f} -> {losses[-1]:.4f}; held-out verdict accuracy "
             f"{acc:.3f}, log-spectrum RMSE {rmse:.3f}")
         model.save(os.path.join(args.out, "surrogate.npz"))
-        summary.update(runs=len(lab), unstable=int(lab.sum()),
+        summary.update(runs=len(lab), focus=args.focus, unstable=int(lab.sum()),
                        data_gpu_s=gpu_s, train_s=train_s,
                        epochs=args.epochs, loss_first=float(losses[0]),
                        loss_last=float(losses[-1]),

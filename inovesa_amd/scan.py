@@ -18,7 +18,8 @@ each round keeps the stable/unstable pair around the first unstable
 current, until the bracket is narrower than rel_tol.
 
 Surrogate-steered scan: the network (surrogate.py) predicts the threshold
-f*; the VFP runs a short geometric grid over f*/(1+w) .. f*(1+w), whose
+f* and the band of currents where its verdict is uncertain; the VFP runs
+one geometric grid over that band (at least f*/(1+w) .. f*(1+w)) whose
 spacing already meets rel_tol; where the transition lies inside the grid
 the scan is done in one round, otherwise the dense rounds continue from the
 bracket the grid leaves.
@@ -187,6 +188,7 @@ class ScanResult:
     runs: int = 0
     surrogate: bool = False
     confirmed: bool = False      # the surrogate's bracket held
+    predicted: float = 0.0       # the surrogate's threshold
 
     @property
     def threshold(self):
@@ -241,14 +243,19 @@ def surrogate_scan(model, targets, setup=None, lo=0.3, hi=3.0, points=16,
     rel_tol in one round); dense rounds continue only where the runs do not
     contain the transition."""
     runner = runner or Runner(setup)
-    if near is None:
-        near = int(np.ceil(2 * np.log1p(window) / np.log1p(rel_tol))) + 1
-    preds = [model.predict_threshold(g, dmp, lo, hi) for g, dmp in targets]
+    preds, wins = [], []
+    for g, dmp in targets:
+        f, (a, b) = model.predict_threshold(g, dmp, lo, hi, band=True)
+        # the window: at least +-window around f*, widened to the band where
+        # the network is unsure (verdict probability 0.1-0.9)
+        a = max(lo, min(f / (1 + window), a))
+        b = min(hi, max(f * (1 + window), b))
+        preds.append(f)
+        wins.append((a, b))
     pts, owner = [], []
-    for k, ((g, dmp), f) in enumerate(zip(targets, preds)):
-        a = max(lo, f / (1 + window))
-        b = min(hi, f * (1 + window))
-        for x in np.geomspace(a, b, near):
+    for k, ((g, dmp), (a, b)) in enumerate(zip(targets, wins)):
+        m = near or int(np.ceil(np.log(b / a) / np.log1p(rel_tol))) + 1
+        for x in np.geomspace(a, b, max(2, m)):
             pts.append((float(x), g, dmp))
             owner.append(k)
     res = runner.run(pts)
@@ -256,7 +263,8 @@ def surrogate_scan(model, targets, setup=None, lo=0.3, hi=3.0, points=16,
     for k, (g, dmp) in enumerate(targets):
         mine = sorted((x for x, o in zip(res, owner) if o == k),
                       key=lambda x: x.factor)
-        r = ScanResult(g, dmp, lo, hi, runs=len(mine), surrogate=True)
+        r = ScanResult(g, dmp, lo, hi, runs=len(mine), surrogate=True,
+                       predicted=preds[k])
         for x in mine:
             if x.unstable:
                 r.hi = x.factor
@@ -265,5 +273,34 @@ def surrogate_scan(model, targets, setup=None, lo=0.3, hi=3.0, points=16,
         # the transition lies between two of the proposed currents
         r.confirmed = (r.lo >= mine[0].factor and r.hi <= mine[-1].factor)
         results.append(r)
+    # a missed target's grid is all stable (go up from its top) or all
+    # unstable (go down from its bottom)
+    _expand(runner, [(r, r.hi >= hi) for r in results if not r.confirmed],
+            points, rel_tol)
     _refine(runner, results, points, rel_tol)
     return results, runner
+
+
+def _expand(runner, missed, points=16, rel_tol=0.02):
+    """One round for the targets whose transition lies outside the proposed
+    grid, given as (result, upwards): `points` currents spaced by
+    (1 + rel_tol) beyond the grid edge the transition lies past, so that a
+    miss of up to (1 + rel_tol)^points is resolved in this round."""
+    pts, owner = [], []
+    for r, up in missed:
+        k = np.arange(1, points + 1) * np.log1p(rel_tol)
+        fs = r.lo * np.exp(k) if up else r.hi * np.exp(-k)
+        for f in fs[(fs > r.lo) & (fs < r.hi)]:
+            pts.append((float(f), r.gap, r.damping))
+            owner.append(r)
+    if not pts:
+        return
+    res = runner.run(pts)
+    for r, _ in missed:
+        mine = [x for x, o in zip(res, owner) if o is r]
+        r.runs += len(mine)
+        for x in sorted(mine, key=lambda x: x.factor):
+            if x.unstable:
+                r.hi = min(r.hi, x.factor)
+                break
+            r.lo = max(r.lo, x.factor)

@@ -21,10 +21,11 @@ from . import vfps as V
 from .scan import Runner, ScanSetup
 
 DIMS = (3, 256, 256, 256, 65)
-# weights trained by `python -m inovesa_amd.pipeline` (4096 runs)
+# weights trained by `python -m inovesa_amd.pipeline` (default arguments)
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                              "data", "surrogate.npz")
 FACTOR_RANGE = (0.3, 3.0)       # current / scaling-law threshold
+FOCUS_RANGE = (0.8, 1.6)        # where the thresholds of the ranges below lie
 GAP_RANGE = (0.016, 0.064)      # full chamber gap (m)
 DAMPING_RANGE = (1000.0, 5000.0)  # damping time (time steps)
 
@@ -42,21 +43,25 @@ def features(factor, gap, damping):
     return x.astype(np.float32)
 
 
-def sample_points(count, seed):
+def sample_points(count, seed, focus=0.0):
     """Training points: current log-uniform, gap uniform, damping
-    log-uniform over the ranges above."""
+    log-uniform over the ranges above; a fraction `focus` of the currents
+    log-uniform over FOCUS_RANGE instead, where the verdict changes."""
     rng = np.random.Generator(np.random.Philox(seed))
     f = np.exp(rng.uniform(*np.log(FACTOR_RANGE), count))
+    near = rng.random(count) < focus
+    f[near] = np.exp(rng.uniform(*np.log(FOCUS_RANGE), int(near.sum())))
     g = rng.uniform(*GAP_RANGE, count)
     dmp = np.exp(rng.uniform(*np.log(DAMPING_RANGE), count))
     return [(float(a), float(b), float(c)) for a, b, c in zip(f, g, dmp)]
 
 
-def generate(count, setup=None, seed=7, chunk=1024, dev=None, log=None):
+def generate(count, setup=None, seed=7, chunk=1024, dev=None, log=None,
+             focus=0.0):
     """Run `count` random points on the batched solver (chunk instances per
     batch); returns (points, spectra [count][64], labels, fluct, gpu s)."""
     runner = Runner(setup or ScanSetup(), dev=dev, spectra=True)
-    pts = sample_points(count, seed)
+    pts = sample_points(count, seed, focus)
     spec, lab, fl = [], [], []
     for k in range(0, count, chunk):
         res = runner.run(pts[k:k + chunk])
@@ -99,20 +104,32 @@ class Surrogate:
         return y[:, :64] * self.std + self.mean, y[:, 64]
 
     def predict_threshold(self, gap, damping, lo=FACTOR_RANGE[0],
-                          hi=FACTOR_RANGE[1], grid=512):
+                          hi=FACTOR_RANGE[1], grid=512, band=False,
+                          p_band=0.1):
         """Current factor where the predicted logit first turns positive
-        (log-linear interpolation on a grid)."""
+        (log-linear interpolation on a grid); with band=True also the
+        factors where the verdict probability first passes p_band and
+        1 - p_band around it."""
         f = np.geomspace(lo, hi, grid)
         _, logit = self.predict(f, gap, damping)
-        pos = np.nonzero(logit > 0)[0]
-        if len(pos) == 0:
-            return float(hi)
-        k = int(pos[0])
-        if k == 0:
-            return float(lo)
-        a, b = logit[k - 1], logit[k]
-        t = a / (a - b)
-        return float(np.exp(np.log(f[k - 1]) + t * (np.log(f[k]) - np.log(f[k - 1]))))
+
+        def crossing(level):
+            pos = np.nonzero(logit > level)[0]
+            if len(pos) == 0:
+                return float(hi)
+            k = int(pos[0])
+            if k == 0:
+                return float(lo)
+            a, b = logit[k - 1] - level, logit[k] - level
+            t = a / (a - b)
+            return float(np.exp(np.log(f[k - 1])
+                                + t * (np.log(f[k]) - np.log(f[k - 1]))))
+
+        f0 = crossing(0.0)
+        if not band:
+            return f0
+        lev = float(np.log(p_band / (1 - p_band)))
+        return f0, (min(crossing(lev), f0), max(crossing(-lev), f0))
 
     def save(self, path):
         np.savez_compressed(path, params=self.net.params(), mean=self.mean,

@@ -30,16 +30,18 @@ def test_surrogate_ablation():
     from inovesa_amd.scan import ScanSetup
     from inovesa_amd.surrogate import Surrogate
     model = Surrogate.load()
-    res = compare(model, scan_targets(8, seed=23), ScanSetup())
-    print(f"\n8 targets: dense {res['dense_runs']} runs "
+    res = compare(model, scan_targets(24, seed=23), ScanSetup())
+    print(f"\n24 targets: dense {res['dense_runs']} runs "
           f"{res['dense_gpu_s']:.2f} s, surrogate {res['surrogate_runs']} runs "
           f"{res['surrogate_gpu_s']:.2f} s (ratio {res['gpu_time_ratio']:.3f}), "
           f"max threshold difference {100 * res['max_rel_diff']:.2f}%, "
           f"{res['confirmed']} confirmed")
     assert res["max_rel_diff"] <= 0.02
-    assert res["surrogate_runs"] <= res["dense_runs"] / 4
-    assert res["confirmed"] >= 6
-    assert res["gpu_time_ratio"] <= 0.5
+    assert res["surrogate_runs"] <= res["dense_runs"] / 5
+    assert res["confirmed"] >= 22
+    # a smaller batch than the 48-target benchmark: the per-step launch
+    # floor weighs more, hence the margin over 1/5
+    assert res["gpu_time_ratio"] <= 0.25
 
 
 def test_surrogate_outputs():

@@ -25,8 +25,9 @@ class FakeModel:
     def __init__(self, thresholds, error):
         self.th, self.err = thresholds, error
 
-    def predict_threshold(self, g, d, lo, hi):
-        return self.th[(g, d)] * (1 + self.err)
+    def predict_threshold(self, g, d, lo, hi, band=False):
+        f = self.th[(g, d)] * (1 + self.err)
+        return (f, (f, f)) if band else f
 
 
 TARGETS = [(0.02, 1500.0), (0.045, 3000.0), (0.06, 4500.0)]
@@ -54,15 +55,38 @@ def test_surrogate_scan_confirms_good_predictions(error):
         assert x.confirmed and x.lo <= t <= x.hi and x.width <= 0.02 + 1e-9
 
 
-@pytest.mark.parametrize("error", [0.15, -0.3])
-def test_surrogate_scan_falls_back(error):
+@pytest.mark.parametrize("error,rounds", [(0.15, 2), (-0.15, 2), (0.25, 2),
+                                          (-0.6, 4), (1.0, 4)])
+def test_surrogate_scan_falls_back(error, rounds):
+    """A miss of up to 1.02^16 = 1.37 beyond the grid costs one expansion
+    round; larger misses continue with the dense rounds."""
     r = FakeRunner(TH)
     res, _ = scan.surrogate_scan(FakeModel(TH, error), TARGETS, runner=r,
                                  rel_tol=0.02)
-    assert r.rounds > 1
+    assert r.rounds == rounds
     for x in res:
         t = TH[(x.gap, x.damping)]
         assert not x.confirmed and x.lo <= t <= x.hi and x.width <= 0.02 + 1e-9
+
+
+class BandModel(FakeModel):
+    """Predicts 8% high but reports a band of +-10% around it."""
+
+    def predict_threshold(self, g, d, lo, hi, band=False):
+        f = self.th[(g, d)] * 1.08
+        return (f, (f / 1.1, f * 1.1)) if band else f
+
+
+def test_surrogate_band_widens_the_grid():
+    r = FakeRunner(TH)
+    res, _ = scan.surrogate_scan(BandModel(TH, 0.0), TARGETS, runner=r,
+                                 rel_tol=0.02)
+    assert r.rounds == 1
+    n_band = int(np.ceil(np.log(1.1 ** 2) / np.log(1.02))) + 1
+    assert r.runs == len(TARGETS) * n_band
+    for x in res:
+        t = TH[(x.gap, x.damping)]
+        assert x.confirmed and x.lo <= t <= x.hi and x.width <= 0.02 + 1e-9
 
 
 def test_detector():
