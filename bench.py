@@ -9,9 +9,12 @@ device every step.  One "step" is one time step of the mesh.
 
 Prints one JSON line.  value = mesh-cell updates per second
 (n^2 * steps / wall time of the timed steps, state resident on the GPU).
-roofline: the step kernel (the whole step in one launch; 8 bytes of mesh
-traffic per cell) timed with HIP events on its stream.  fft_wake: the same
-workload with the wake and CSR power from the FFT field kernel.  batched:
+The x projection is summed in the reference's sequential order, bit for
+bit (--projection wavefront: as a wavefront reduction; other_projection
+reports the order not chosen).  roofline: the step kernel (the whole step in one
+launch; 8 bytes of mesh traffic per cell) timed with HIP events on its
+stream.  fft_wake: the same workload with the wake and CSR power from the
+FFT field kernel.  batched:
 64 independent meshes of 128^2-512^2 in one launch per step.  scan: the
 threshold scan with and without the trained network.  cpu_baseline: the
 oracle (the FP32 C restatement of the reference CPU path) on one host
@@ -128,6 +131,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--projection", choices=("wavefront", "ordered"),
+                    default="ordered",
+                    help="x projection: wavefront reduction, or the "
+                         "reference's sequential order (bit-exact)")
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--scan-targets", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -151,7 +158,12 @@ def main():
                                 current_factor=1.5)
     zw, zc = impedances(d)
     init = initial_mesh(n, seed=1)
-    st = Step(desc, init, zw, zc)
+    # x projection in the reference's sequential order (bit-exact; the
+    # wake amplifies a last-bit change of the profile some 300 times, so
+    # only this order keeps the per-step wake within 1e-5) or as a
+    # wavefront reduction
+    fast = 1 if args.projection == "wavefront" else 0
+    st = Step(dict(desc, fast_projection=fast), init, zw, zc)
     st.record(args.warmup + args.steps, profiles=True)
     st.run(args.warmup)
     st.dev.sync()
@@ -176,8 +188,22 @@ def main():
                 "kernel_ms": ms_step_kernel,
                 "bytes_per_launch": bytes_per_launch}
 
+    # the same workload with the other projection order
+    other = Step(dict(desc, fast_projection=1 - fast), init, zw, zc)
+    other.run(640)
+    other.dev.sync()
+    ko = min(args.steps, 12800)
+    t0 = time.perf_counter()
+    other.run(ko)
+    other.dev.sync()
+    dto = time.perf_counter() - t0
+    other_line = {"projection": "ordered" if fast else "wavefront",
+                  "value": n * n * ko / dto, "steps": ko,
+                  "ms_per_step": dto * 1e3 / ko}
+    del other
+
     # the same workload with the FFT wake (field kernel + step kernel)
-    fft = Step(dict(desc, wake_method=1), init, zw, zc)
+    fft = Step(dict(desc, wake_method=1, fast_projection=fast), init, zw, zc)
     fft.run(640)
     fft.dev.sync()
     kf = min(args.steps, 6400)
@@ -202,9 +228,12 @@ def main():
                                   "recorded every step",
                       "n": n, "nmax": zw.n, "steps_per_ts": 2000,
                       "Ib_A": d["Ib"], "wake": "direct (FP64 Toeplitz)",
+                      "x_projection": ("wavefront reduction" if fast else
+                                       "sequential, reference order"),
                       "tile_rows": cfg["tile_rows"],
                       "threads": cfg["threads"]},
-           "roofline": roofline, "fft_wake": fft_line}
+           "roofline": roofline, "fft_wake": fft_line,
+           "other_projection": other_line}
     if not args.no_batch:
         out["batched"] = batched_throughput(args.batch_steps, log)
     if not args.no_scan:

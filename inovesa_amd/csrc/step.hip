@@ -31,40 +31,47 @@ namespace vfps_impl {
 // ---- fused step kernel ---------------------------------------------------------
 
 template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP>
-static const void* step_fn()
+static const void* step_fn(bool single)
 {
-    return reinterpret_cast<const void*>(&step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP>);
+    if (single) {
+        return reinterpret_cast<const void*>(
+            &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, true>);
+    }
+    return reinterpret_cast<const void*>(
+        &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, false>);
 }
 
 template <int IT, int IPFP, bool DIRECT>
-static const void* step_fn_scheme(const StepVariant& v)
+static const void* step_fn_scheme(const StepVariant& v, bool single)
 {
-    if (v.scheme == 0) return step_fn<IT, IPFP, DIRECT, 0, false>();
-    if (v.clamp) return step_fn<IT, IPFP, DIRECT, 1, true>();
-    return step_fn<IT, IPFP, DIRECT, 1, false>();
+    if (v.scheme == 0) return step_fn<IT, IPFP, DIRECT, 0, false>(single);
+    if (v.clamp) return step_fn<IT, IPFP, DIRECT, 1, true>(single);
+    return step_fn<IT, IPFP, DIRECT, 1, false>(single);
 }
 
 template <int IT, bool DIRECT>
-static const void* step_fn_fp(const StepVariant& v)
+static const void* step_fn_fp(const StepVariant& v, bool single)
 {
     switch (v.ipfp) {
-    case 0: return step_fn_scheme<IT, 0, DIRECT>(v);
-    case 3: return step_fn_scheme<IT, 3, DIRECT>(v);
-    case 4: return step_fn_scheme<IT, 4, DIRECT>(v);
-    case 5: return step_fn_scheme<IT, 5, DIRECT>(v);
+    case 0: return step_fn_scheme<IT, 0, DIRECT>(v, single);
+    case 3: return step_fn_scheme<IT, 3, DIRECT>(v, single);
+    case 4: return step_fn_scheme<IT, 4, DIRECT>(v, single);
+    case 5: return step_fn_scheme<IT, 5, DIRECT>(v, single);
     }
     return nullptr;
 }
 
 // resolve a variant to its kernel (cubic and linear interpolation; the
-// clamp needs cubic interpolation)
-const void* step_kernel_for(const StepVariant& v)
+// clamp needs cubic interpolation); single: the one-mesh form
+const void* step_kernel_for(const StepVariant& v, bool single)
 {
     if (v.it == 4) {
-        return v.direct ? step_fn_fp<4, true>(v) : step_fn_fp<4, false>(v);
+        return v.direct ? step_fn_fp<4, true>(v, single)
+                        : step_fn_fp<4, false>(v, single);
     }
     if (v.it == 2 && !v.clamp) {
-        return v.direct ? step_fn_fp<2, true>(v) : step_fn_fp<2, false>(v);
+        return v.direct ? step_fn_fp<2, true>(v, single)
+                        : step_fn_fp<2, false>(v, single);
     }
     return nullptr;
 }
@@ -77,15 +84,17 @@ size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
 
 int step_set_lds_limit(const StepVariant& v, size_t bytes)
 {
-    const void* fn = step_kernel_for(v);
-    if (!fn) {
-        return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
-                                          "interpolation/stencil combination");
-    }
-    if (bytes > 65536) {
-        VFPS_HIP(hipFuncSetAttribute(fn,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)bytes));
+    for (bool single : {false, true}) {
+        const void* fn = step_kernel_for(v, single);
+        if (!fn) {
+            return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
+                                              "interpolation/stencil combination");
+        }
+        if (bytes > 65536) {
+            VFPS_HIP(hipFuncSetAttribute(fn,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)bytes));
+        }
     }
     return VFPS_OK;
 }
@@ -93,14 +102,24 @@ int step_set_lds_limit(const StepVariant& v, size_t bytes)
 int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
                        const uint32_t* blockmap_d, uint32_t nblocks,
                        size_t lds_bytes, uint32_t threads, uint32_t parity,
-                       hipStream_t st)
+                       hipStream_t st, const InstDesc* desc_h)
 {
-    const void* fn = step_kernel_for(v);
+    const bool single = desc_h != nullptr;
+    if (single && blockmap_d) {
+        return fail(VFPS_ERR_INVALID, "fused step: a by-value descriptor "
+                                  "serves one mesh, without a block map");
+    }
+    const void* fn = step_kernel_for(v, single);
     if (!fn) {
         return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
                                           "interpolation/stencil combination");
     }
-    void* args[] = {(void*)&descs_d, (void*)&blockmap_d, (void*)&parity};
+    InstDesc d0{};
+    if (single) {
+        d0 = *desc_h;
+    }
+    void* args[] = {(void*)&descs_d, (void*)&blockmap_d, (void*)&parity,
+                    (void*)&d0};
     VFPS_HIP(hipLaunchKernel(fn, dim3(nblocks), dim3(threads), args, lds_bytes,
                              st));
     return VFPS_OK;
@@ -280,7 +299,7 @@ static int step_launch_fused(vfps_step* s, uint32_t parity, hipStream_t st)
     return launch_step_kernel(s->variant, s->inst_d, nullptr, s->tiles(),
                               step_lds_bytes(s->variant, s->desc.n, s->T,
                                              s->stride),
-                              s->threads, parity, st);
+                              s->threads, parity, st, &s->inst_h);
 }
 
 static int step_launch_one(vfps_step* s, uint32_t parity)
@@ -407,6 +426,7 @@ static int step_upload_desc(vfps_step* s)
     d.T = s->T;
     d.stride = s->stride;
     VFPS_HIP(hipMemcpy(s->inst_d, &d, sizeof(InstDesc), hipMemcpyHostToDevice));
+    s->inst_h = d;
 
     FieldParams p{};
     p.nb = 1;
@@ -953,6 +973,7 @@ int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
     unsigned long long* fstamp = buf + 8;
     VFPS_HIP(hipMemcpy(reinterpret_cast<char*>(s->inst_d) + offsetof(InstDesc, tstamp),
                        &mstamp, sizeof(mstamp), hipMemcpyHostToDevice));
+    s->inst_h.tstamp = mstamp;
     VFPS_HIP(hipMemcpy(reinterpret_cast<char*>(s->fparams_d) + offsetof(FieldParams, tstamp),
                        &fstamp, sizeof(fstamp), hipMemcpyHostToDevice));
     int rc = step_launch_one(s, s->parity);
@@ -964,6 +985,7 @@ int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
     unsigned long long* nul = nullptr;
     hipMemcpy(reinterpret_cast<char*>(s->inst_d) + offsetof(InstDesc, tstamp),
               &nul, sizeof(nul), hipMemcpyHostToDevice);
+    s->inst_h.tstamp = nullptr;
     hipMemcpy(reinterpret_cast<char*>(s->fparams_d) + offsetof(FieldParams, tstamp),
               &nul, sizeof(nul), hipMemcpyHostToDevice);
     hipFree(buf);
@@ -1032,7 +1054,7 @@ int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
             rc = launch_step_kernel(s->variant, s->inst_d, nullptr, s->tiles(),
                                     step_lds_bytes(s->variant, s->desc.n, s->T,
                                                    s->stride),
-                                    s->threads, s->parity, st);
+                                    s->threads, s->parity, st, &s->inst_h);
             s->parity ^= 1u;
         }
         hipEventRecord(e1, st);
@@ -1053,7 +1075,7 @@ int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
                                         s->tiles(),
                                         step_lds_bytes(s->variant, s->desc.n,
                                                        s->T, s->stride),
-                                        s->threads, par, st);
+                                        s->threads, par, st, &s->inst_h);
             }
             hipEventRecord(e2, st);
             hipEventSynchronize(e2);

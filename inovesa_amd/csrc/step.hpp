@@ -55,7 +55,7 @@ struct StepVariant {
     bool direct, clamp;
 };
 
-const void* step_kernel_for(const StepVariant& v);
+const void* step_kernel_for(const StepVariant& v, bool single = false);
 size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
                       uint32_t stride);
 int step_set_lds_limit(const StepVariant& v, size_t bytes);
@@ -63,10 +63,12 @@ int step_set_lds_limit(const StepVariant& v, size_t bytes);
 // `lds_bytes` of dynamic LDS (at least step_lds_bytes of every instance);
 // blockmap_d (optional) gives (instance << 16 | tile) per logical
 // workgroup (STEP_NO_TILE: padding), else instance 0, tile = workgroup.
+// desc_h (optional, no block map): the one mesh's descriptor, passed to the
+// kernel by value instead of read from descs_d.
 int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
                        const uint32_t* blockmap_d, uint32_t nblocks,
                        size_t lds_bytes, uint32_t threads, uint32_t parity,
-                       hipStream_t st);
+                       hipStream_t st, const InstDesc* desc_h = nullptr);
 // FP64 Toeplitz kernels of the direct wake (g) and CSR power (h) forms
 void toeplitz_tables(uint32_t n, uint32_t M, const float2* Z,
                      const float2* Zc, const float* renorm, float fdelta,
@@ -116,6 +118,7 @@ struct vfps_step {
     uint32_t* csr_valid = nullptr;  // [2]
     float* csr_power = nullptr;
     vfps_impl::InstDesc* inst_d = nullptr;
+    vfps_impl::InstDesc inst_h{};     // host copy, passed by value at launch
     vfps_impl::FieldParams* fparams_d = nullptr;
     vfps_impl::StepVariant variant{};
     uint32_t T = 1, threads = 256, stride = 1, hmax = 0, M = 0;

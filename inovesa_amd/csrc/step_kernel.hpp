@@ -205,10 +205,14 @@ __device__ __forceinline__ void kick_rows(const float* __restrict__ src,
     }
 }
 
-template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP>
+// SINGLE: one mesh, its descriptor passed by value (the launch's own
+// arguments, so no load of a descriptor or block map precedes the data
+// loads); else descs[] and blockmap[] of a batch.
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, bool SINGLE>
 __global__ void __launch_bounds__(STEP_MAX_THREADS)
 step_kernel(const InstDesc* __restrict__ descs,
-            const uint32_t* __restrict__ blockmap, uint32_t parity)
+            const uint32_t* __restrict__ blockmap, uint32_t parity,
+            InstDesc d0)
 {
     extern __shared__ __align__(16) float lds[];
     __shared__ double red[16];
@@ -225,7 +229,7 @@ step_kernel(const InstDesc* __restrict__ descs,
         logical = (blockIdx.x & 7u) * (nblocks >> 3) + (blockIdx.x >> 3);
     }
     uint32_t inst = 0, tile = logical;
-    if (blockmap) {
+    if (!SINGLE && blockmap) {
         const uint32_t m = blockmap[logical];
         if (m == STEP_NO_TILE) {
             return;  // padding of the grid to a multiple of the XCD count
@@ -233,7 +237,13 @@ step_kernel(const InstDesc* __restrict__ descs,
         inst = m >> 16;
         tile = m & 0xFFFFu;
     }
-    const InstDesc& d = descs[inst];
+    const InstDesc* dp;
+    if constexpr (SINGLE) {
+        dp = &d0;
+    } else {
+        dp = descs + inst;
+    }
+    const InstDesc& d = *dp;
     const uint32_t n = d.n;
     const uint32_t T = d.T;
     const uint32_t stride = d.stride;
@@ -603,7 +613,42 @@ step_kernel(const InstDesc* __restrict__ descs,
             const float* r = tile_out + (size_t)tid * tstride;
             float acc = 0.0f;
             uint32_t y = 0;
-            if ((n & 15u) == 0u) {
+            if ((n & 63u) == 0u) {
+                // two register buffers of 32 values: one loads while the
+                // other is added, so 32 dependent adds cover each LDS trip
+                const vec4f* r4 = reinterpret_cast<const vec4f*>(r);
+                vec4f qa[8], qb[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    qa[k] = r4[k];
+                }
+                for (; y < n; y += 64) {
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        qb[k] = r4[((y + 32) >> 2) + k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        acc = acc + qa[k].x;
+                        acc = acc + qa[k].y;
+                        acc = acc + qa[k].z;
+                        acc = acc + qa[k].w;
+                    }
+                    // the last pass reloads the row's first values (unused)
+                    const uint32_t yn = y + 64 < n ? y + 64 : 0u;
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        qa[k] = r4[(yn >> 2) + k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        acc = acc + qb[k].x;
+                        acc = acc + qb[k].y;
+                        acc = acc + qb[k].z;
+                        acc = acc + qb[k].w;
+                    }
+                }
+            } else if ((n & 15u) == 0u) {
                 // the next 16 values load while the current 16 are added
                 const vec4f* r4 = reinterpret_cast<const vec4f*>(r);
                 vec4f q[4], nq[4];

@@ -88,18 +88,37 @@ def test_fused_equals_per_map_path(V, scheme, n, kw):
     assert len(ca) == 70 and np.array_equal(ca, cb) and np.array_equal(fa, fb)
 
 
-def test_fast_projection_mode(V, orc):
-    desc, d, zw, zc, init = _setup(V, 0, 512)
-    st = V.Step(dict(desc, fast_projection=1), init, zw, zc)
-    st.run(1)
-    mesh, proj, wake = st.mesh()
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("scheme,n,kw", CASES)
+def test_fast_projection_mode(V, orc, scheme, n, kw, method):
+    """The x projection as a wavefront reduction (lane-strided partial sums,
+    then a butterfly) instead of the reference's sequential order: the
+    profile differs in its last bits and the first step stays within the
+    per-step bound.  A second step from the same mesh takes its wake from
+    the tree-summed profile on the GPU and from the ordered one in the
+    oracle; the wake amplifies that last-bit difference (3.5e-5 at 512^2),
+    which is why the ordered sum is the default."""
+    desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
+    st = V.Step(dict(desc, fast_projection=1, wake_method=method), init,
+                zw, zc)
     sim = orc.sim(oracle_cfg(desc, zw.n), init, zw.data(), zc.data())
+    st.run(1)
     sim.run(1)
+    mesh, proj, wake = st.mesh()
     omesh, oproj, owake = sim.get()
-    # the tree-summed profile differs in its last bits; the mesh stays close
-    assert rel_linf(proj, oproj) <= 1e-5
+    assert rel_linf(proj, oproj) <= 1e-6
     assert rel_linf(mesh, omesh) <= 1e-5
+    assert rel_linf(wake, owake) <= 1e-5
+    sim.set_mesh(mesh.reshape(n, n))
+    st.run(1)
+    sim.run(1)
+    mesh, proj, wake = st.mesh()
+    omesh, oproj, owake = sim.get()
+    print(f"\nscheme {scheme} n={n} wake method {method}, second step: "
+          f"wake {rel_linf(wake, owake):.2e} mesh {rel_linf(mesh, omesh):.2e}")
     assert rel_linf(wake, owake) <= 1e-4
+    assert rel_linf(mesh, omesh) <= 1e-5
+    assert rel_linf(proj, oproj) <= 1e-6
 
 
 def test_graph_replay_deterministic(V):
@@ -156,14 +175,16 @@ def test_tile_variants_identical_fft(V, tile, threads):
     assert np.array_equal(st.mesh()[0], ref.mesh()[0])
 
 
+@pytest.mark.parametrize("fast", [0, 1])
 @pytest.mark.parametrize("scheme,n,kw", CASES[:2])
 @pytest.mark.parametrize("tile,threads", TILE_VARIANTS)
-def test_tile_variants_identical_direct(V, scheme, n, kw, tile, threads):
+def test_tile_variants_identical_direct(V, scheme, n, kw, tile, threads, fast):
     """Every tile/workgroup variant gives the same mesh, profile and wake
-    bit for bit (direct wake)."""
+    bit for bit (direct wake; either projection order)."""
     if threads * 4 < n:
         pytest.skip("too few threads for the mesh")
     desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
+    desc = dict(desc, fast_projection=fast)
     ref = V.Step(dict(desc, wake_method=2, tile_rows=1, threads=256), init,
                  zw, zc)
     ref.run(5)
@@ -174,12 +195,14 @@ def test_tile_variants_identical_direct(V, scheme, n, kw, tile, threads):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("fast", [0, 1])
 @pytest.mark.parametrize("scheme,n,kw", [(0, 256, {"steps_per_ts": 500}),
-                                         (1, 128, {"steps_per_ts": 300})])
-def test_history_and_drift_vs_fp64(V, orc, orc64, scheme, n, kw):
+                                         (1, 128, {"steps_per_ts": 300}),
+                                         (0, 512, {})])
+def test_history_and_drift_vs_fp64(V, orc, orc64, scheme, n, kw, fast):
     steps = 100
     desc, d, zw, zc, init = _setup(V, scheme, n, **kw)
-    st = V.Step(desc, init, zw, zc)
+    st = V.Step(dict(desc, fast_projection=fast), init, zw, zc)
     st.record(steps, profiles=True)
     st.run(steps)
     csr, prof = st.history()
@@ -198,7 +221,8 @@ def test_history_and_drift_vs_fp64(V, orc, orc64, scheme, n, kw):
     e32 = rel_linf(mesh, omesh)
     e64 = rel_linf(mesh, m64)
     o64 = rel_linf(omesh, m64)
-    print(f"\n{steps} steps, scheme {scheme}, n={n}: GPU vs FP32 oracle "
+    print(f"\n{steps} steps, scheme {scheme}, n={n}, fast projection "
+          f"{fast}: GPU vs FP32 oracle "
           f"{e32:.2e}, GPU vs FP64 {e64:.2e}, FP32 oracle vs FP64 {o64:.2e}")
     assert e32 <= 1e-4
     assert e64 <= 10 * o64 + 1e-5
