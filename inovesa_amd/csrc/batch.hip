@@ -25,6 +25,8 @@ namespace {
 struct Group {
     StepVariant variant{};
     uint32_t threads = 256;
+    uint32_t lds_class = 0;         // LDS bytes of a tile, in 8 KiB units
+    uint32_t maxn = 0;              // largest mesh of the group
     std::vector<uint32_t> members;  // batch indices
     InstDesc* descs_d = nullptr;
     uint32_t* blockmap_d = nullptr;
@@ -96,8 +98,10 @@ int build_group(vfps_batch* b, Group& g)
     });
     std::vector<uint32_t> map;
     g.lds = 0;
+    g.maxn = 0;
     for (uint32_t k : order) {
         const vfps_step* s = b->members[g.members[k]];
+        g.maxn = std::max(g.maxn, s->desc.n);
         for (uint32_t t = 0; t < s->tiles(); t++) {
             map.push_back((k << 16) | t);
         }
@@ -124,7 +128,10 @@ int build_group(vfps_batch* b, Group& g)
     return VFPS_OK;
 }
 
-// (re)build the launch groups from the members' current configuration
+// (re)build the launch groups from the members' current configuration:
+// one launch per kernel variant, workgroup size and LDS size class (every
+// workgroup of a launch gets the largest tile's LDS, so mixing small and
+// large tiles would cost the small ones occupancy)
 int build_groups(vfps_batch* b)
 {
     int rc = drop_graph(b);
@@ -132,9 +139,13 @@ int build_groups(vfps_batch* b)
     free_groups(b);
     for (uint32_t i = 0; i < b->members.size(); i++) {
         const vfps_step* s = b->members[i];
+        const uint32_t cls = ceil_div(step_lds_bytes(s->variant, s->desc.n,
+                                                     s->T, s->stride),
+                                      8192);
         Group* g = nullptr;
         for (auto& e : b->groups) {
-            if (same_variant(e.variant, s->variant) && e.threads == s->threads) {
+            if (same_variant(e.variant, s->variant) && e.threads == s->threads &&
+                e.lds_class == cls) {
                 g = &e;
                 break;
             }
@@ -144,6 +155,7 @@ int build_groups(vfps_batch* b)
             g = &b->groups.back();
             g->variant = s->variant;
             g->threads = s->threads;
+            g->lds_class = cls;
         }
         g->members.push_back(i);
     }
@@ -172,7 +184,8 @@ int launch_step(vfps_batch* b, uint32_t parity)
             VFPS_HIP(hipStreamWaitEvent(st, b->fork, 0));
         }
         int rc = launch_step_kernel(g.variant, g.descs_d, g.blockmap_d,
-                                    g.nblocks, g.lds, g.threads, parity, st);
+                                    g.nblocks, g.lds, g.threads, g.maxn,
+                                    parity, st);
         if (rc != VFPS_OK) return rc;
         if (k > 0) {
             VFPS_HIP(hipEventRecord(g.done, st));

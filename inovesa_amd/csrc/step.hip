@@ -30,48 +30,64 @@ namespace vfps_impl {
 
 // ---- fused step kernel ---------------------------------------------------------
 
-template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP>
-static const void* step_fn(bool single)
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, bool SINGLE>
+static const void* step_fn_cols(uint32_t maxc)
 {
-    if (single) {
+    if (maxc <= 1) {
         return reinterpret_cast<const void*>(
-            &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, true>);
+            &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, SINGLE, 1>);
+    }
+    if (maxc == 2) {
+        return reinterpret_cast<const void*>(
+            &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, SINGLE, 2>);
     }
     return reinterpret_cast<const void*>(
-        &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, false>);
+        &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, SINGLE, STEP_MAX_COLS>);
+}
+
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP>
+static const void* step_fn(bool single, uint32_t maxc)
+{
+    if (single) return step_fn_cols<IT, IPFP, DIRECT, SCHEME, CLAMP, true>(maxc);
+    return step_fn_cols<IT, IPFP, DIRECT, SCHEME, CLAMP, false>(maxc);
 }
 
 template <int IT, int IPFP, bool DIRECT>
-static const void* step_fn_scheme(const StepVariant& v, bool single)
+static const void* step_fn_scheme(const StepVariant& v, bool single,
+                                  uint32_t maxc)
 {
-    if (v.scheme == 0) return step_fn<IT, IPFP, DIRECT, 0, false>(single);
-    if (v.clamp) return step_fn<IT, IPFP, DIRECT, 1, true>(single);
-    return step_fn<IT, IPFP, DIRECT, 1, false>(single);
+    if (v.scheme == 0) return step_fn<IT, IPFP, DIRECT, 0, false>(single, maxc);
+    if (v.clamp) return step_fn<IT, IPFP, DIRECT, 1, true>(single, maxc);
+    return step_fn<IT, IPFP, DIRECT, 1, false>(single, maxc);
 }
 
 template <int IT, bool DIRECT>
-static const void* step_fn_fp(const StepVariant& v, bool single)
+static const void* step_fn_fp(const StepVariant& v, bool single, uint32_t maxc)
 {
     switch (v.ipfp) {
-    case 0: return step_fn_scheme<IT, 0, DIRECT>(v, single);
-    case 3: return step_fn_scheme<IT, 3, DIRECT>(v, single);
-    case 4: return step_fn_scheme<IT, 4, DIRECT>(v, single);
-    case 5: return step_fn_scheme<IT, 5, DIRECT>(v, single);
+    case 0: return step_fn_scheme<IT, 0, DIRECT>(v, single, maxc);
+    case 3: return step_fn_scheme<IT, 3, DIRECT>(v, single, maxc);
+    case 4: return step_fn_scheme<IT, 4, DIRECT>(v, single, maxc);
+    case 5: return step_fn_scheme<IT, 5, DIRECT>(v, single, maxc);
     }
     return nullptr;
 }
 
 // resolve a variant to its kernel (cubic and linear interpolation; the
-// clamp needs cubic interpolation); single: the one-mesh form
-const void* step_kernel_for(const StepVariant& v, bool single)
+// clamp needs cubic interpolation); single: the one-mesh form; maxc: most
+// columns per thread (1, 2, else up to STEP_MAX_COLS)
+const void* step_kernel_for(const StepVariant& v, bool single, uint32_t maxc)
 {
+    if (maxc > (uint32_t)STEP_MAX_COLS) {
+        return nullptr;
+    }
     if (v.it == 4) {
-        return v.direct ? step_fn_fp<4, true>(v, single)
-                        : step_fn_fp<4, false>(v, single);
+        return v.direct ? step_fn_fp<4, true>(v, single, maxc)
+                        : step_fn_fp<4, false>(v, single, maxc);
     }
     if (v.it == 2 && !v.clamp) {
-        return v.direct ? step_fn_fp<2, true>(v, single)
-                        : step_fn_fp<2, false>(v, single);
+        return v.direct ? step_fn_fp<2, true>(v, single, maxc)
+                        : step_fn_fp<2, false>(v, single, maxc);
     }
     return nullptr;
 }
@@ -79,21 +95,24 @@ const void* step_kernel_for(const StepVariant& v, bool single)
 size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
                       uint32_t stride)
 {
-    return sizeof(float) * step_lds(n, T, stride, v.direct, (uint32_t)v.it).words;
+    return sizeof(float) *
+           step_lds(n, T, stride, v.direct, (uint32_t)v.it, v.scheme).words;
 }
 
 int step_set_lds_limit(const StepVariant& v, size_t bytes)
 {
     for (bool single : {false, true}) {
-        const void* fn = step_kernel_for(v, single);
-        if (!fn) {
-            return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
-                                              "interpolation/stencil combination");
-        }
-        if (bytes > 65536) {
-            VFPS_HIP(hipFuncSetAttribute(fn,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)bytes));
+        for (uint32_t maxc : {1u, 2u, (uint32_t)STEP_MAX_COLS}) {
+            const void* fn = step_kernel_for(v, single, maxc);
+            if (!fn) {
+                return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for "
+                                                  "this interpolation/stencil "
+                                                  "combination");
+            }
+            if (bytes > 65536) {
+                VFPS_HIP(hipFuncSetAttribute(
+                    fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+            }
         }
     }
     return VFPS_OK;
@@ -101,15 +120,18 @@ int step_set_lds_limit(const StepVariant& v, size_t bytes)
 
 int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
                        const uint32_t* blockmap_d, uint32_t nblocks,
-                       size_t lds_bytes, uint32_t threads, uint32_t parity,
-                       hipStream_t st, const InstDesc* desc_h)
+                       size_t lds_bytes, uint32_t threads, uint32_t maxn,
+                       uint32_t parity, hipStream_t st, const InstDesc* desc_h)
 {
     const bool single = desc_h != nullptr;
     if (single && blockmap_d) {
         return fail(VFPS_ERR_INVALID, "fused step: a by-value descriptor "
                                   "serves one mesh, without a block map");
     }
-    const void* fn = step_kernel_for(v, single);
+    if (threads == 0 || threads > STEP_MAX_THREADS || threads % 64u != 0u) {
+        return fail(VFPS_ERR_INVALID, "fused step: workgroup size");
+    }
+    const void* fn = step_kernel_for(v, single, ceil_div(maxn, threads));
     if (!fn) {
         return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
                                           "interpolation/stencil combination");
@@ -299,7 +321,7 @@ static int step_launch_fused(vfps_step* s, uint32_t parity, hipStream_t st)
     return launch_step_kernel(s->variant, s->inst_d, nullptr, s->tiles(),
                               step_lds_bytes(s->variant, s->desc.n, s->T,
                                              s->stride),
-                              s->threads, parity, st, &s->inst_h);
+                              s->threads, s->desc.n, parity, st, &s->inst_h);
 }
 
 static int step_launch_one(vfps_step* s, uint32_t parity)
@@ -1054,7 +1076,8 @@ int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
             rc = launch_step_kernel(s->variant, s->inst_d, nullptr, s->tiles(),
                                     step_lds_bytes(s->variant, s->desc.n, s->T,
                                                    s->stride),
-                                    s->threads, s->parity, st, &s->inst_h);
+                                    s->threads, s->desc.n, s->parity, st,
+                                    &s->inst_h);
             s->parity ^= 1u;
         }
         hipEventRecord(e1, st);
@@ -1075,7 +1098,8 @@ int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
                                         s->tiles(),
                                         step_lds_bytes(s->variant, s->desc.n,
                                                        s->T, s->stride),
-                                        s->threads, par, st, &s->inst_h);
+                                        s->threads, s->desc.n, par, st,
+                                        &s->inst_h);
             }
             hipEventRecord(e2, st);
             hipEventSynchronize(e2);

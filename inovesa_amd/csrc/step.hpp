@@ -55,7 +55,8 @@ struct StepVariant {
     bool direct, clamp;
 };
 
-const void* step_kernel_for(const StepVariant& v, bool single = false);
+const void* step_kernel_for(const StepVariant& v, bool single = false,
+                            uint32_t maxc = 1);
 size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
                       uint32_t stride);
 int step_set_lds_limit(const StepVariant& v, size_t bytes);
@@ -63,12 +64,14 @@ int step_set_lds_limit(const StepVariant& v, size_t bytes);
 // `lds_bytes` of dynamic LDS (at least step_lds_bytes of every instance);
 // blockmap_d (optional) gives (instance << 16 | tile) per logical
 // workgroup (STEP_NO_TILE: padding), else instance 0, tile = workgroup.
+// maxn: the largest mesh size of the launch (sets the columns per thread).
 // desc_h (optional, no block map): the one mesh's descriptor, passed to the
 // kernel by value instead of read from descs_d.
 int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
                        const uint32_t* blockmap_d, uint32_t nblocks,
-                       size_t lds_bytes, uint32_t threads, uint32_t parity,
-                       hipStream_t st, const InstDesc* desc_h = nullptr);
+                       size_t lds_bytes, uint32_t threads, uint32_t maxn,
+                       uint32_t parity, hipStream_t st,
+                       const InstDesc* desc_h = nullptr);
 // FP64 Toeplitz kernels of the direct wake (g) and CSR power (h) forms
 void toeplitz_tables(uint32_t n, uint32_t M, const float2* Z,
                      const float2* Zc, const float* renorm, float fdelta,

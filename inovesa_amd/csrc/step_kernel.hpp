@@ -49,7 +49,9 @@ struct StepLds {
     uint32_t pprev;               // previous x projection
     uint32_t row0;                // input row 0 (scheme 1 out-of-range taps)
     uint32_t bufA, bufB;          // row buffers, `stride` rows each
-    uint32_t tile_out;            // [T][n + 4] projection products
+    uint32_t tile_out;            // [T][n + 4] projection products, or
+                                  // STEP_TILE_IN_BUF: in the row buffer the
+                                  // drift/rotation read (dead by then)
     uint32_t offw, offr;          // per-row kick offsets
     uint32_t tapw, tapr;          // per-row taps: IT shifts, then IT weights
     uint32_t csr_rows;            // FP64 CSR share of each output row
@@ -66,8 +68,13 @@ __host__ __device__ inline uint32_t step_round4(uint32_t v) { return (v + 3u) & 
 constexpr uint32_t STEP_TAP_WORDS = 8;
 
 
+constexpr uint32_t STEP_TILE_IN_BUF = 0xFFFFFFFFu;
+
+// scheme 1 (rotation) with the direct wake also keeps input row 0 and the
+// kernel window of its wake offset
 __host__ __device__ inline StepLds step_lds(uint32_t n, uint32_t T, uint32_t stride,
-                                            bool direct, uint32_t it)
+                                            bool direct, uint32_t it,
+                                            int scheme)
 {
     StepLds L{};
     uint32_t w = 0;
@@ -78,20 +85,28 @@ __host__ __device__ inline StepLds step_lds(uint32_t n, uint32_t T, uint32_t str
         w += 2 * (n + T);
         L.csr_rows = w;
         w += 2 * T;
-        L.g0win = w;
-        w += 2 * n;
+        if (scheme == 1) {
+            L.g0win = w;
+            w += 2 * n;
+        }
         w = step_round4(w);
         L.pprev = w;
         w += step_round4(n);
-        L.row0 = w;
-        w += step_round4(n);
+        if (scheme == 1) {
+            L.row0 = w;
+            w += step_round4(n);
+        }
     }
     L.bufA = w;
     w += step_round4(stride * n);
     L.bufB = w;
     w += step_round4(stride * n);
-    L.tile_out = w;
-    w += T * (n + 4);
+    if ((size_t)stride * n >= (size_t)T * (n + 4)) {
+        L.tile_out = STEP_TILE_IN_BUF;
+    } else {
+        L.tile_out = w;
+        w += T * (n + 4);
+    }
     L.offw = w;
     w += step_round4(stride);
     L.offr = w;
@@ -208,7 +223,10 @@ __device__ __forceinline__ void kick_rows(const float* __restrict__ src,
 // SINGLE: one mesh, its descriptor passed by value (the launch's own
 // arguments, so no load of a descriptor or block map precedes the data
 // loads); else descs[] and blockmap[] of a batch.
-template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, bool SINGLE>
+// MAXC: most mesh columns one thread handles (ceil(n / threads) of every
+// instance of the launch), which sizes the per-column register tables.
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, bool SINGLE,
+          int MAXC>
 __global__ void __launch_bounds__(STEP_MAX_THREADS)
 step_kernel(const InstDesc* __restrict__ descs,
             const uint32_t* __restrict__ blockmap, uint32_t parity,
@@ -259,10 +277,11 @@ step_kernel(const InstDesc* __restrict__ descs,
     // rows of the buffer that exist in the mesh
     const int lr0 = max(0, -rbase);
     const int lr1 = min(R, (int)n - rbase);
-    const StepLds L = step_lds(n, T, stride, DIRECT, IT);
+    const StepLds L = step_lds(n, T, stride, DIRECT, IT, SCHEME);
     float* bufA = lds + L.bufA;
     float* bufB = lds + L.bufB;
-    float* tile_out = lds + L.tile_out;
+    // placed after the drift/rotation, which frees the buffer it read
+    float* tile_out = nullptr;
     float* offw = lds + L.offw;
     float* offr = lds + L.offr;
     float* tapw = lds + L.tapw;
@@ -340,13 +359,13 @@ step_kernel(const InstDesc* __restrict__ descs,
     }
     //  - per-column tables in registers
     const int ncols = (int)((n + bs - 1) / bs);
-    float col_ws[STEP_MAX_COLS];
-    uint32_t col_didx[STEP_MAX_COLS][IT];   // drift taps of the column
-    float col_dw[STEP_MAX_COLS][IT];
-    uint32_t col_fi[STEP_MAX_COLS][IPFP > 0 ? IPFP : 1];
-    float col_fw[STEP_MAX_COLS][IPFP > 0 ? IPFP : 1];
+    float col_ws[MAXC];
+    uint32_t col_didx[MAXC][IT];   // drift taps of the column
+    float col_dw[MAXC][IT];
+    uint32_t col_fi[MAXC][IPFP > 0 ? IPFP : 1];
+    float col_fw[MAXC][IPFP > 0 ? IPFP : 1];
 #pragma unroll
-    for (int c = 0; c < STEP_MAX_COLS; c++) {
+    for (int c = 0; c < MAXC; c++) {
         const uint32_t y = tid + (uint32_t)c * bs;
         if (c < ncols && y < n) {
             if (SCHEME == 0) {
@@ -489,7 +508,7 @@ step_kernel(const InstDesc* __restrict__ descs,
         VFPS_STAMP(4);
         // ---- E: drift along x, one column per thread ----------------------
 #pragma unroll
-        for (int c = 0; c < STEP_MAX_COLS; c++) {
+        for (int c = 0; c < MAXC; c++) {
             const uint32_t y = tid + (uint32_t)c * bs;
             if (c >= ncols || y >= n) {
                 continue;
@@ -571,9 +590,10 @@ step_kernel(const InstDesc* __restrict__ descs,
     __syncthreads();
     VFPS_STAMP(5);
     // ---- F: Fokker-Planck along y, store, projection products ----------------
+    tile_out = L.tile_out == STEP_TILE_IN_BUF ? cur : lds + L.tile_out;
     const uint32_t tstride = n + 4;
 #pragma unroll
-    for (int c = 0; c < STEP_MAX_COLS; c++) {
+    for (int c = 0; c < MAXC; c++) {
         const uint32_t y = tid + (uint32_t)c * bs;
         if (c >= ncols || y >= n) {
             continue;

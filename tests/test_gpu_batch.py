@@ -45,7 +45,10 @@ def test_batch_equals_single_steps(V, kick_drift):
     b = _batch(V, inst)
     info = b.info()
     assert info["instances"] == len(inst)
-    assert info["launch_groups"] == (2 if kick_drift else 1)
+    # one launch per kernel variant, workgroup size and LDS size class:
+    # at least one per variant, at most one per (variant, mesh size)
+    variants = 2 if kick_drift else 1
+    assert variants <= info["launch_groups"] <= 3 + (1 if kick_drift else 0)
     steps = 70                       # one graph replay and 6 single steps
     b.record(steps + 8)
     b.run(steps)
