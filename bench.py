@@ -77,7 +77,9 @@ def batched_throughput(steps, log):
     b.run(steps)
     b.dev.sync()
     dt = time.perf_counter() - t0
-    chosen = {r["n"]: (r["T"], r["threads"], round(r["us_per_step"], 2))
+    chosen = {r["n"]: (r["T"], r["threads"],
+                       "persistent" if r["persistent"] else "per-tile",
+                       round(r["us_per_step"], 2))
               for r in table if r["chosen"]}
     log(f"batched: {len(cfg)} instances, {cells} cells, {steps} steps in "
         f"{dt:.2f} s")

@@ -368,7 +368,8 @@ VFPS_API int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
  * nanoseconds since the kernel's first stamp; [15] the clock period */
 VFPS_API int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout);
 /* the configuration in use: out[0] fused, [1] wake method (1 FFT, 2 direct),
- * [2] tile rows, [3] threads, [4] LDS buffer rows, [5] LDS bytes (nout >= 6) */
+ * [2] tile rows, [3] threads, [4] LDS buffer rows, [5] LDS bytes (nout >= 6),
+ * [6] persistent launch in a batch (nout >= 7) */
 VFPS_API int vfps_step_config(vfps_step* s, uint32_t* out, uint32_t nout);
 
 /* ---- batch of independent meshes ------------------------------------- *
@@ -402,17 +403,31 @@ VFPS_API int vfps_batch_get_csr_histories(vfps_batch* b, uint32_t first,
 /* instances, launch groups and mesh cells per step */
 VFPS_API int vfps_batch_info(vfps_batch* b, uint32_t* ninst,
                              uint32_t* ngroups, uint64_t* cells);
-/* vfps_step_config of one instance */
+/* the launches of one step, cap rows of 4: workgroup size, tiles in the
+ * block map (with padding), workgroups launched (fewer than the tiles:
+ * persistent), dynamic LDS bytes; *ngroups = number of launches */
+VFPS_API int vfps_batch_launches(vfps_batch* b, uint32_t* out, uint32_t cap,
+                                 uint32_t* ngroups);
+/* Set the tiling of one instance by hand: tile rows, workgroup size and
+ * launch form (persistent != 0: as many workgroups as stay resident, each
+ * walking several tiles).  Results do not depend on it.  On failure the
+ * previous tiling stays. */
+VFPS_API int vfps_batch_retile(vfps_batch* b, uint32_t inst, uint32_t tile_rows,
+                               uint32_t threads, int persistent);
+/* vfps_step_config of one instance (out[6], if nout >= 7: persistent) */
 VFPS_API int vfps_batch_config(vfps_batch* b, uint32_t inst, uint32_t* out,
                                uint32_t nout);
 /* average time of one batch step (all launches), HIP events on the device
  * stream; advances the state by nsteps */
 VFPS_API int vfps_batch_timing(vfps_batch* b, uint32_t nsteps, float* ms_step);
-/* Autotuner: per mesh size, time every tile height T in {1,2,4,8,16} and
- * workgroup size in {128,256,512} that fits (steps steps each, on that
- * size's instances alone) and keep the fastest.  The instances' state is
- * restored afterwards; results do not depend on the choice.  table (cap
- * rows of 5 floats: n, T, threads, us/step, chosen) may be NULL. */
+/* Autotuner: per mesh size, time every tile height T in {1,2,4,8,16},
+ * workgroup size in {128,256,512} and launch form (one workgroup per tile,
+ * or persistent: as many workgroups as stay resident, each walking several
+ * tiles) that fits (steps steps each, on that size's instances alone) and
+ * keep the fastest.  The instances' state is restored afterwards; results
+ * do not depend on the choice.  table (cap rows of VFPS_TUNE_COLS floats:
+ * n, T, threads, us/step, chosen, persistent) may be NULL. */
+#define VFPS_TUNE_COLS 6
 VFPS_API int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                                  uint32_t cap, uint32_t* nrows);
 

@@ -27,6 +27,8 @@ struct Group {
     uint32_t threads = 256;
     uint32_t lds_class = 0;         // LDS bytes of a tile, in 8 KiB units
     uint32_t maxn = 0;              // largest mesh of the group
+    bool persist = false;           // grid sized to the resident workgroups
+    uint32_t grid = 0;              // workgroups (0: one per map entry)
     std::vector<uint32_t> members;  // batch indices
     InstDesc* descs_d = nullptr;
     uint32_t* blockmap_d = nullptr;
@@ -114,6 +116,7 @@ int build_group(vfps_batch* b, Group& g)
     VFPS_CHECK_ARG(g.lds <= b->dev->lds_max,
                    "batch: a tile exceeds the LDS of one CU");
     g.nblocks = (uint32_t)map.size();
+    g.grid = 0;
     VFPS_HIP(hipMalloc(&g.blockmap_d, sizeof(uint32_t) * map.size()));
     VFPS_HIP(hipMemcpy(g.blockmap_d, map.data(), sizeof(uint32_t) * map.size(),
                        hipMemcpyHostToDevice));
@@ -124,6 +127,17 @@ int build_group(vfps_batch* b, Group& g)
     }
     int rc = step_set_lds_limit(g.variant, g.lds);
     if (rc != VFPS_OK) return rc;
+    if (g.persist) {
+        // as many workgroups as stay resident on the device at once, a
+        // multiple of 8 (one share per XCD), each walking several tiles
+        int per_cu = 0;
+        rc = step_occupancy(g.variant, g.threads, g.maxn, g.lds, &per_cu);
+        if (rc != VFPS_OK) return rc;
+        const uint32_t res = (uint32_t)std::max(per_cu, 1) *
+                             (uint32_t)std::max(b->dev->ncu, 8);
+        const uint32_t grid = std::min(g.nblocks, res) & ~7u;
+        g.grid = grid < g.nblocks ? std::max(grid, 8u) : 0u;
+    }
     VFPS_HIP(hipEventCreateWithFlags(&g.done, hipEventDisableTiming));
     return VFPS_OK;
 }
@@ -145,7 +159,7 @@ int build_groups(vfps_batch* b)
         Group* g = nullptr;
         for (auto& e : b->groups) {
             if (same_variant(e.variant, s->variant) && e.threads == s->threads &&
-                e.lds_class == cls) {
+                e.lds_class == cls && e.persist == s->persist) {
                 g = &e;
                 break;
             }
@@ -156,6 +170,7 @@ int build_groups(vfps_batch* b)
             g->variant = s->variant;
             g->threads = s->threads;
             g->lds_class = cls;
+            g->persist = s->persist;
         }
         g->members.push_back(i);
     }
@@ -185,7 +200,7 @@ int launch_step(vfps_batch* b, uint32_t parity)
         }
         int rc = launch_step_kernel(g.variant, g.descs_d, g.blockmap_d,
                                     g.nblocks, g.lds, g.threads, g.maxn,
-                                    parity, st);
+                                    parity, st, nullptr, g.grid);
         if (rc != VFPS_OK) return rc;
         if (k > 0) {
             VFPS_HIP(hipEventRecord(g.done, st));
@@ -405,12 +420,58 @@ int vfps_batch_info(vfps_batch* b, uint32_t* ninst, uint32_t* ngroups,
     return VFPS_OK;
 }
 
+int vfps_batch_launches(vfps_batch* b, uint32_t* out, uint32_t cap,
+                        uint32_t* ngroups)
+{
+    VFPS_CHECK_ARG(b && (out || cap == 0), "vfps_batch_launches: bad argument");
+    for (size_t k = 0; k < b->groups.size() && k < cap; k++) {
+        const Group& g = b->groups[k];
+        uint32_t* r = out + 4 * k;
+        r[0] = g.threads;
+        r[1] = g.nblocks;
+        r[2] = g.grid ? g.grid : g.nblocks;
+        r[3] = (uint32_t)g.lds;
+    }
+    if (ngroups) *ngroups = (uint32_t)b->groups.size();
+    return VFPS_OK;
+}
+
 int vfps_batch_config(vfps_batch* b, uint32_t inst, uint32_t* out,
                       uint32_t nout)
 {
     VFPS_CHECK_ARG(b && inst < b->members.size() && out && nout >= 6,
                    "vfps_batch_config: bad argument");
     return vfps_step_config(b->members[inst], out, nout);
+}
+
+int vfps_batch_retile(vfps_batch* b, uint32_t inst, uint32_t tile_rows,
+                      uint32_t threads, int persistent)
+{
+    VFPS_CHECK_ARG(b && inst < b->members.size(), "vfps_batch_retile: bad "
+                                                  "instance");
+    VFPS_CHECK_ARG(tile_rows >= 1 && threads >= 64 && threads % 64 == 0 &&
+                       threads <= STEP_MAX_THREADS,
+                   "vfps_batch_retile: tile rows or workgroup size");
+    VFPS_TRY_BEGIN
+    VFPS_HIP(hipStreamSynchronize(b->dev->stream));
+    sync_parity(b);
+    vfps_step* s = b->members[inst];
+    const uint32_t T0 = s->T, thr0 = s->threads;
+    const bool p0 = s->persist;
+    s->persist = persistent != 0;
+    int rc = step_retile(s, tile_rows, threads);
+    if (rc == VFPS_OK) {
+        rc = build_groups(b);
+    }
+    if (rc != VFPS_OK) {
+        // keep the batch runnable with the previous tiling
+        s->persist = p0;
+        step_retile(s, T0, thr0);
+        build_groups(b);
+        return rc;
+    }
+    return VFPS_OK;
+    VFPS_TRY_END
 }
 
 int vfps_batch_timing(vfps_batch* b, uint32_t nsteps, float* ms_step)
@@ -485,68 +546,84 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
         float best = 1e30f;
         uint32_t bestT = b->members[cl.second[0]]->T;
         uint32_t bestThr = b->members[cl.second[0]]->threads;
+        bool bestPers = b->members[cl.second[0]]->persist;
         const uint32_t first_row = rows;
         for (uint32_t thr : {128u, 256u, 512u}) {
             for (uint32_t T : {1u, 2u, 4u, 8u, 16u}) {
-                if (ceil_div(n, thr) > (uint32_t)STEP_MAX_COLS || T > n) continue;
-                bool ok = true;
-                for (uint32_t i : cl.second) {
-                    if (step_retile(b->members[i], T, thr) != VFPS_OK) {
-                        ok = false;
-                        break;
+                for (bool pers : {false, true}) {
+                    if (ceil_div(n, thr) > (uint32_t)STEP_MAX_COLS || T > n) continue;
+                    bool ok = true;
+                    for (uint32_t i : cl.second) {
+                        b->members[i]->persist = pers;
+                        if (step_retile(b->members[i], T, thr) != VFPS_OK) {
+                            ok = false;
+                            break;
+                        }
+                    }
+                    if (!ok) continue;
+                    // a batch of this class only
+                    vfps_batch sub;
+                    sub.dev = b->dev;
+                    sub.fork = b->fork;
+                    for (uint32_t i : cl.second) sub.members.push_back(b->members[i]);
+                    rc = build_groups(&sub);
+                    // a persistent grid as large as the map is the plain launch
+                    bool dup = false;
+                    if (rc == VFPS_OK && pers) {
+                        dup = true;
+                        for (const auto& g : sub.groups) dup = dup && g.grid == 0;
+                    }
+                    float ms = 0;
+                    if (rc == VFPS_OK && !dup) {
+                        // warm up, then time
+                        for (uint32_t k = 0; k < 2 && rc == VFPS_OK; k++) {
+                            rc = launch_step(&sub, (par + k) & 1u);
+                        }
+                        hipEventRecord(e0, st);
+                        for (uint32_t k = 0; k < even && rc == VFPS_OK; k++) {
+                            rc = launch_step(&sub, (par + k) & 1u);
+                        }
+                        hipEventRecord(e1, st);
+                        hipEventSynchronize(e1);
+                        hipEventElapsedTime(&ms, e0, e1);
+                    }
+                    free_groups(&sub);
+                    sub.fork = nullptr;
+                    sub.members.clear();
+                    if (rc != VFPS_OK) break;
+                    if (dup) continue;
+                    const float us = 1e3f * ms / (float)even;
+                    if (table && rows < cap) {
+                        float* r = table + VFPS_TUNE_COLS * (size_t)rows;
+                        r[0] = (float)n;
+                        r[1] = (float)T;
+                        r[2] = (float)thr;
+                        r[3] = us;
+                        r[4] = 0.0f;
+                        r[5] = pers ? 1.0f : 0.0f;
+                    }
+                    rows++;
+                    if (us < best) {
+                        best = us;
+                        bestT = T;
+                        bestThr = thr;
+                        bestPers = pers;
                     }
                 }
-                if (!ok) continue;
-                // a batch of this class only
-                vfps_batch sub;
-                sub.dev = b->dev;
-                sub.fork = b->fork;
-                for (uint32_t i : cl.second) sub.members.push_back(b->members[i]);
-                rc = build_groups(&sub);
-                float ms = 0;
-                if (rc == VFPS_OK) {
-                    // warm up, then time
-                    for (uint32_t k = 0; k < 2 && rc == VFPS_OK; k++) {
-                        rc = launch_step(&sub, (par + k) & 1u);
-                    }
-                    hipEventRecord(e0, st);
-                    for (uint32_t k = 0; k < even && rc == VFPS_OK; k++) {
-                        rc = launch_step(&sub, (par + k) & 1u);
-                    }
-                    hipEventRecord(e1, st);
-                    hipEventSynchronize(e1);
-                    hipEventElapsedTime(&ms, e0, e1);
-                }
-                free_groups(&sub);
-                sub.fork = nullptr;
-                sub.members.clear();
                 if (rc != VFPS_OK) break;
-                const float us = 1e3f * ms / (float)even;
-                if (table && rows < cap) {
-                    float* r = table + 5 * (size_t)rows;
-                    r[0] = (float)n;
-                    r[1] = (float)T;
-                    r[2] = (float)thr;
-                    r[3] = us;
-                    r[4] = 0.0f;
-                }
-                rows++;
-                if (us < best) {
-                    best = us;
-                    bestT = T;
-                    bestThr = thr;
-                }
             }
             if (rc != VFPS_OK) break;
         }
         if (rc != VFPS_OK) break;
         for (uint32_t i : cl.second) {
+            b->members[i]->persist = bestPers;
             if ((rc = step_retile(b->members[i], bestT, bestThr)) != VFPS_OK) break;
         }
         if (table) {
             for (uint32_t r = first_row; r < rows && r < cap; r++) {
-                float* row = table + 5 * (size_t)r;
-                if ((uint32_t)row[1] == bestT && (uint32_t)row[2] == bestThr) {
+                float* row = table + VFPS_TUNE_COLS * (size_t)r;
+                if ((uint32_t)row[1] == bestT && (uint32_t)row[2] == bestThr &&
+                    (row[5] != 0.0f) == bestPers) {
                     row[4] = 1.0f;
                 }
             }

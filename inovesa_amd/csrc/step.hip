@@ -121,12 +121,26 @@ int step_set_lds_limit(const StepVariant& v, size_t bytes)
 int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
                        const uint32_t* blockmap_d, uint32_t nblocks,
                        size_t lds_bytes, uint32_t threads, uint32_t maxn,
-                       uint32_t parity, hipStream_t st, const InstDesc* desc_h)
+                       uint32_t parity, hipStream_t st, const InstDesc* desc_h,
+                       uint32_t grid)
 {
     const bool single = desc_h != nullptr;
-    if (single && blockmap_d) {
-        return fail(VFPS_ERR_INVALID, "fused step: a by-value descriptor "
-                                  "serves one mesh, without a block map");
+    if (single == (blockmap_d != nullptr)) {
+        return fail(VFPS_ERR_INVALID, "fused step: one mesh takes a by-value "
+                                      "descriptor, a batch a block map");
+    }
+    if (grid == 0) {
+        grid = nblocks;
+    }
+    if (!single && (nblocks % 8u != 0u || grid % 8u != 0u || grid == 0 ||
+                    grid > nblocks)) {
+        return fail(VFPS_ERR_INVALID, "fused step: a batch's block map and "
+                                      "grid must be multiples of 8, the grid "
+                                      "at most the map");
+    }
+    if (single && grid != nblocks) {
+        return fail(VFPS_ERR_INVALID, "fused step: one mesh runs one "
+                                      "workgroup per tile");
     }
     if (threads == 0 || threads > STEP_MAX_THREADS || threads % 64u != 0u) {
         return fail(VFPS_ERR_INVALID, "fused step: workgroup size");
@@ -140,10 +154,25 @@ int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
     if (single) {
         d0 = *desc_h;
     }
-    void* args[] = {(void*)&descs_d, (void*)&blockmap_d, (void*)&parity,
-                    (void*)&d0};
-    VFPS_HIP(hipLaunchKernel(fn, dim3(nblocks), dim3(threads), args, lds_bytes,
+    void* args[] = {(void*)&descs_d, (void*)&blockmap_d, (void*)&nblocks,
+                    (void*)&parity, (void*)&d0};
+    VFPS_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds_bytes,
                              st));
+    return VFPS_OK;
+}
+
+int step_occupancy(const StepVariant& v, uint32_t threads, uint32_t maxn,
+                   size_t lds_bytes, int* per_cu)
+{
+    const void* fn = step_kernel_for(v, false, ceil_div(maxn, threads));
+    if (!fn) {
+        return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
+                                          "interpolation/stencil combination");
+    }
+    int occ = 0;
+    VFPS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, (int)threads,
+                                                          lds_bytes));
+    *per_cu = occ;
     return VFPS_OK;
 }
 
@@ -976,6 +1005,9 @@ int vfps_step_config(vfps_step* s, uint32_t* out, uint32_t nout)
     out[5] = s->fused ? (uint32_t)step_lds_bytes(s->variant, s->desc.n, s->T,
                                                  s->stride)
                       : 0u;
+    if (nout >= 7) {
+        out[6] = s->persist ? 1u : 0u;
+    }
     return VFPS_OK;
 }
 

@@ -65,13 +65,18 @@ int step_set_lds_limit(const StepVariant& v, size_t bytes);
 // blockmap_d (optional) gives (instance << 16 | tile) per logical
 // workgroup (STEP_NO_TILE: padding), else instance 0, tile = workgroup.
 // maxn: the largest mesh size of the launch (sets the columns per thread).
-// desc_h (optional, no block map): the one mesh's descriptor, passed to the
-// kernel by value instead of read from descs_d.
+// One mesh: desc_h (its descriptor, passed to the kernel by value), no
+// block map, one workgroup per tile.  A batch: blockmap_d of nblocks
+// entries (a multiple of 8); grid (a multiple of 8, at most nblocks; 0:
+// nblocks) workgroups walk it, several tiles each when grid < nblocks.
 int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
                        const uint32_t* blockmap_d, uint32_t nblocks,
                        size_t lds_bytes, uint32_t threads, uint32_t maxn,
                        uint32_t parity, hipStream_t st,
-                       const InstDesc* desc_h = nullptr);
+                       const InstDesc* desc_h = nullptr, uint32_t grid = 0);
+// workgroups of this launch shape that stay resident on one CU
+int step_occupancy(const StepVariant& v, uint32_t threads, uint32_t maxn,
+                   size_t lds_bytes, int* per_cu);
 // FP64 Toeplitz kernels of the direct wake (g) and CSR power (h) forms
 void toeplitz_tables(uint32_t n, uint32_t M, const float2* Z,
                      const float2* Zc, const float* renorm, float fdelta,
@@ -125,6 +130,7 @@ struct vfps_step {
     vfps_impl::FieldParams* fparams_d = nullptr;
     vfps_impl::StepVariant variant{};
     uint32_t T = 1, threads = 256, stride = 1, hmax = 0, M = 0;
+    bool persist = false;  // in a batch: persistent launch (grid < tiles)
     bool fused = true;
     bool direct = true;
     uint32_t parity = 0;  // fused path: buffer holding the state (0: t1)

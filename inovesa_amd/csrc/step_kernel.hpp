@@ -220,48 +220,29 @@ __device__ __forceinline__ void kick_rows(const float* __restrict__ src,
     }
 }
 
-// SINGLE: one mesh, its descriptor passed by value (the launch's own
-// arguments, so no load of a descriptor or block map precedes the data
-// loads); else descs[] and blockmap[] of a batch.
-// MAXC: most mesh columns one thread handles (ceil(n / threads) of every
-// instance of the launch), which sizes the per-column register tables.
-template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, bool SINGLE,
-          int MAXC>
-__global__ void __launch_bounds__(STEP_MAX_THREADS)
-step_kernel(const InstDesc* __restrict__ descs,
-            const uint32_t* __restrict__ blockmap, uint32_t parity,
-            InstDesc d0)
+// Workgroup-shared scalars of one tile
+struct StepShared {
+    double red[16];
+    float s00;
+    uint32_t slot;
+    uint32_t cnt;
+};
+
+// One row tile of one instance: phases A-H.  Every early return is uniform
+// over the workgroup.  MAXC: most mesh columns one thread handles
+// (ceil(n / threads) of every instance of the launch), which sizes the
+// per-column register tables.
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, int MAXC>
+__device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
+                                          uint32_t tile, uint32_t logical,
+                                          uint32_t parity, float* lds,
+                                          StepShared& sh)
 {
-    extern __shared__ __align__(16) float lds[];
-    __shared__ double red[16];
-    __shared__ float s00_sh;
-    __shared__ uint32_t slot_sh;
-    __shared__ uint32_t cnt_sh;
+    double* red = sh.red;
+    float& s00_sh = sh.s00;
+    uint32_t& slot_sh = sh.slot;
+    uint32_t& cnt_sh = sh.cnt;
     const unsigned long long t_entry = wall_clock64();
-    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
-    // so give each XCD a contiguous run of tiles (neighbouring tiles share
-    // halo rows, which then hit in that XCD's L2).
-    const uint32_t nblocks = gridDim.x;
-    uint32_t logical = blockIdx.x;
-    if ((nblocks & 7u) == 0u) {
-        logical = (blockIdx.x & 7u) * (nblocks >> 3) + (blockIdx.x >> 3);
-    }
-    uint32_t inst = 0, tile = logical;
-    if (!SINGLE && blockmap) {
-        const uint32_t m = blockmap[logical];
-        if (m == STEP_NO_TILE) {
-            return;  // padding of the grid to a multiple of the XCD count
-        }
-        inst = m >> 16;
-        tile = m & 0xFFFFu;
-    }
-    const InstDesc* dp;
-    if constexpr (SINGLE) {
-        dp = &d0;
-    } else {
-        dp = descs + inst;
-    }
-    const InstDesc& d = *dp;
     const uint32_t n = d.n;
     const uint32_t T = d.T;
     const uint32_t stride = d.stride;
@@ -765,7 +746,7 @@ step_kernel(const InstDesc* __restrict__ descs,
         }
     }
     if (d.tstamp) {
-        // per-workgroup entry and exit times
+        // per-tile entry and exit times
         __syncthreads();
         if (threadIdx.x == 0) {
             d.tstamp[16 + 2 * logical] = t_entry;
@@ -775,5 +756,52 @@ step_kernel(const InstDesc* __restrict__ descs,
 }
 
 #undef VFPS_STAMP
+
+// The step of one launch.
+//  SINGLE: one mesh, its descriptor passed by value (the launch's own
+//  arguments, so no load of a descriptor or block map precedes the data
+//  loads), one workgroup per tile.
+//  Else a batch: blockmap[nmap] holds (instance << 16 | tile) per logical
+//  tile (STEP_NO_TILE: padding), nmap and the grid multiples of 8.
+// XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so the
+// map is cut into 8 contiguous runs, one per XCD (neighbouring tiles share
+// halo rows, which then hit in that XCD's L2).  The workgroups of an XCD
+// walk its run: one tile each when the grid holds the whole map, several
+// in a persistent launch (a smaller grid, sized to what stays resident).
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, bool SINGLE,
+          int MAXC>
+__global__ void __launch_bounds__(STEP_MAX_THREADS)
+step_kernel(const InstDesc* __restrict__ descs,
+            const uint32_t* __restrict__ blockmap, uint32_t nmap,
+            uint32_t parity, InstDesc d0)
+{
+    extern __shared__ __align__(16) float lds[];
+    __shared__ StepShared sh;
+    const uint32_t G = gridDim.x;
+    if constexpr (SINGLE) {
+        uint32_t logical = blockIdx.x;
+        if ((G & 7u) == 0u) {
+            logical = (blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+        }
+        step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC>(d0, 0u, logical,
+                                                         logical, parity, lds,
+                                                         sh);
+    } else {
+        const uint32_t per = nmap >> 3;
+        const uint32_t wgs = G >> 3;
+        const uint32_t xcd = blockIdx.x & 7u;
+        for (uint32_t i = blockIdx.x >> 3; i < per; i += wgs) {
+            const uint32_t logical = xcd * per + i;
+            const uint32_t m = blockmap[logical];
+            if (m == STEP_NO_TILE) {
+                continue;
+            }
+            step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC>(
+                descs[m >> 16], m >> 16, m & 0xFFFFu, logical, parity, lds, sh);
+            // the next tile reuses the LDS
+            __syncthreads();
+        }
+    }
+}
 
 }  // namespace vfps_impl

@@ -683,10 +683,28 @@ class Batch:
                     cells=c.value)
 
     def config(self, i):
-        out = np.zeros(6, np.uint32)
-        call("vfps_batch_config", self.h, i, _p(out), 6)
+        """Instance i's configuration: fused, wake method, tile rows,
+        threads, LDS buffer rows, LDS bytes, persistent launch."""
+        out = np.zeros(7, np.uint32)
+        call("vfps_batch_config", self.h, i, _p(out), 7)
         return dict(zip(("fused", "wake_method", "tile_rows", "threads",
-                         "stride", "lds_bytes"), (int(v) for v in out)))
+                         "stride", "lds_bytes", "persistent"),
+                        (int(v) for v in out)))
+
+    def launches(self):
+        """The launches of one step: threads, tiles, workgroups, LDS bytes
+        (fewer workgroups than tiles: a persistent launch)."""
+        cap = 64
+        out = np.zeros((cap, 4), np.uint32)
+        ng = C.c_uint32()
+        call("vfps_batch_launches", self.h, _p(out), cap, C.byref(ng))
+        return [dict(threads=int(r[0]), tiles=int(r[1]), workgroups=int(r[2]),
+                     lds_bytes=int(r[3])) for r in out[:min(ng.value, cap)]]
+
+    def retile(self, i, tile_rows, threads, persistent=False):
+        """Set instance i's tiling by hand (results do not depend on it)."""
+        call("vfps_batch_retile", self.h, i, tile_rows, threads,
+             1 if persistent else 0)
 
     def timing(self, nsteps):
         ms = C.c_float()
@@ -694,16 +712,18 @@ class Batch:
         return ms.value
 
     def autotune(self, steps=20):
-        """Pick the tile height and workgroup size per mesh size; returns the
-        timing table (n, T, threads, us/step, chosen)."""
-        cap = 64
-        table = np.zeros((cap, 5), np.float32)
+        """Pick the tile height, workgroup size and launch form (persistent
+        or one workgroup per tile) per mesh size; returns the timing table
+        (n, T, threads, us/step, chosen, persistent)."""
+        cap = 512
+        table = np.zeros((cap, 6), np.float32)
         rows = C.c_uint32()
         call("vfps_batch_autotune", self.h, steps, _p(table), cap,
              C.byref(rows))
         return [dict(n=int(r[0]), T=int(r[1]), threads=int(r[2]),
-                     us_per_step=float(r[3]), chosen=bool(r[4]))
-                for r in table[:rows.value]]
+                     us_per_step=float(r[3]), chosen=bool(r[4]),
+                     persistent=bool(r[5]))
+                for r in table[:min(rows.value, cap)]]
 
     def __del__(self):
         h = getattr(self, "h", None)

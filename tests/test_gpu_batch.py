@@ -63,6 +63,46 @@ def test_batch_equals_single_steps(V, kick_drift):
         assert len(cb) == steps and np.array_equal(cb, cs), k
 
 
+@pytest.mark.parametrize("tile,threads", [(1, 128), (1, 256), (1, 512)])
+def test_persistent_launch_equals_single_steps(V, tile, threads):
+    """The persistent form (as many workgroups as stay resident, each
+    walking several tiles of the ragged map) gives every instance's result
+    bit for bit."""
+    inst = _instances(V, (128, 256, 384, 512), 4)
+    b = _batch(V, inst)
+    for k in range(len(inst)):
+        b.retile(k, tile, threads, persistent=True)
+        assert b.config(k)["persistent"] == 1
+    shapes = b.launches()
+    print("\nlaunches (threads, tiles, workgroups, LDS):",
+          [tuple(x.values()) for x in shapes])
+    # at least one launch walks several tiles per workgroup
+    assert any(x["workgroups"] < x["tiles"] for x in shapes)
+    assert all(x["workgroups"] % 8 == 0 for x in shapes)
+    steps = 66                       # one graph replay and 2 single steps
+    b.record(steps + 4)
+    b.run(steps)
+    for k in (0, 5, 9, len(inst) - 1):
+        s = _single(V, inst[k])
+        s.record(steps + 4)
+        s.run(steps)
+        for x, y in zip(b.mesh(k), s.mesh()):
+            assert np.array_equal(x, y), k
+        cb, _ = b.history(k)
+        cs, _ = s.history()
+        assert np.array_equal(cb, cs), k
+
+
+def test_retile_rejects_and_keeps_tiling(V):
+    inst = _instances(V, (128,), 2)
+    b = _batch(V, inst)
+    before = b.config(0)
+    with pytest.raises(V.VfpsInvalid):
+        b.retile(0, 1, 100)
+    assert b.config(0) == before
+    b.run(3)
+
+
 def test_autotune_keeps_results_and_reports_table(V):
     inst = _instances(V, (128, 256, 384, 512), 1)
     a = _batch(V, inst)
@@ -77,11 +117,12 @@ def test_autotune_keeps_results_and_reports_table(V):
         assert len(rows) >= 3
         chosen = [r for r in rows if r["chosen"]]
         assert len(chosen) == 1
+        assert all(r["persistent"] in (False, True) for r in rows)
         assert chosen[0]["us_per_step"] == min(r["us_per_step"] for r in rows)
         k = [j for j, i in enumerate(inst) if i[0]["n"] == n][0]
         cfg = a.config(k)
-        assert (cfg["tile_rows"], cfg["threads"]) == (chosen[0]["T"],
-                                                      chosen[0]["threads"])
+        assert (cfg["tile_rows"], cfg["threads"], cfg["persistent"]) == (
+            chosen[0]["T"], chosen[0]["threads"], int(chosen[0]["persistent"]))
     print("\nautotune table (n, T, threads, us/step, chosen):")
     for r in table:
         print(f"  {r['n']:4d} {r['T']:2d} {r['threads']:4d} "
