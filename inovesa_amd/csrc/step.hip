@@ -25,69 +25,39 @@
 #include "reduce.hpp"
 #include "step.hpp"
 #include "step_kernel.hpp"
+#include "step_select.hpp"
 
 namespace vfps_impl {
 
 // ---- fused step kernel ---------------------------------------------------------
 
-template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, bool SINGLE>
-static const void* step_fn_cols(uint32_t maxc)
-{
-    if (maxc <= 1) {
-        return reinterpret_cast<const void*>(
-            &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, SINGLE, 1>);
-    }
-    if (maxc == 2) {
-        return reinterpret_cast<const void*>(
-            &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, SINGLE, 2>);
-    }
-    return reinterpret_cast<const void*>(
-        &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, SINGLE, STEP_MAX_COLS>);
-}
-
-template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP>
-static const void* step_fn(bool single, uint32_t maxc)
-{
-    if (single) return step_fn_cols<IT, IPFP, DIRECT, SCHEME, CLAMP, true>(maxc);
-    return step_fn_cols<IT, IPFP, DIRECT, SCHEME, CLAMP, false>(maxc);
-}
-
-template <int IT, int IPFP, bool DIRECT>
-static const void* step_fn_scheme(const StepVariant& v, bool single,
-                                  uint32_t maxc)
-{
-    if (v.scheme == 0) return step_fn<IT, IPFP, DIRECT, 0, false>(single, maxc);
-    if (v.clamp) return step_fn<IT, IPFP, DIRECT, 1, true>(single, maxc);
-    return step_fn<IT, IPFP, DIRECT, 1, false>(single, maxc);
-}
-
 template <int IT, bool DIRECT>
-static const void* step_fn_fp(const StepVariant& v, bool single, uint32_t maxc)
+static const void* step_fn_fp(const StepVariant& v, int form, uint32_t maxc)
 {
     switch (v.ipfp) {
-    case 0: return step_fn_scheme<IT, 0, DIRECT>(v, single, maxc);
-    case 3: return step_fn_scheme<IT, 3, DIRECT>(v, single, maxc);
-    case 4: return step_fn_scheme<IT, 4, DIRECT>(v, single, maxc);
-    case 5: return step_fn_scheme<IT, 5, DIRECT>(v, single, maxc);
+    case 0: return step_kernels<IT, DIRECT, 0>(v, form, maxc);
+    case 3: return step_kernels<IT, DIRECT, 3>(v, form, maxc);
+    case 4: return step_kernels<IT, DIRECT, 4>(v, form, maxc);
+    case 5: return step_kernels<IT, DIRECT, 5>(v, form, maxc);
     }
     return nullptr;
 }
 
 // resolve a variant to its kernel (cubic and linear interpolation; the
-// clamp needs cubic interpolation); single: the one-mesh form; maxc: most
-// columns per thread (1, 2, else up to STEP_MAX_COLS)
-const void* step_kernel_for(const StepVariant& v, bool single, uint32_t maxc)
+// clamp needs cubic interpolation); form: StepForm; maxc: most columns per
+// thread (1, 2, else up to STEP_MAX_COLS)
+const void* step_kernel_for(const StepVariant& v, int form, uint32_t maxc)
 {
     if (maxc > (uint32_t)STEP_MAX_COLS) {
         return nullptr;
     }
     if (v.it == 4) {
-        return v.direct ? step_fn_fp<4, true>(v, single, maxc)
-                        : step_fn_fp<4, false>(v, single, maxc);
+        return v.direct ? step_fn_fp<4, true>(v, form, maxc)
+                        : step_fn_fp<4, false>(v, form, maxc);
     }
     if (v.it == 2 && !v.clamp) {
-        return v.direct ? step_fn_fp<2, true>(v, single, maxc)
-                        : step_fn_fp<2, false>(v, single, maxc);
+        return v.direct ? step_fn_fp<2, true>(v, form, maxc)
+                        : step_fn_fp<2, false>(v, form, maxc);
     }
     return nullptr;
 }
@@ -101,9 +71,12 @@ size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
 
 int step_set_lds_limit(const StepVariant& v, size_t bytes)
 {
-    for (bool single : {false, true}) {
+    for (int form : {STEP_SINGLE, STEP_PER_TILE, STEP_PERSISTENT}) {
         for (uint32_t maxc : {1u, 2u, (uint32_t)STEP_MAX_COLS}) {
-            const void* fn = step_kernel_for(v, single, maxc);
+            const void* fn = step_kernel_for(v, form, maxc);
+            if (!fn && form != STEP_SINGLE && !v.direct) {
+                continue;  // batch forms exist for the direct wake only
+            }
             if (!fn) {
                 return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for "
                                                   "this interpolation/stencil "
@@ -145,7 +118,9 @@ int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
     if (threads == 0 || threads > STEP_MAX_THREADS || threads % 64u != 0u) {
         return fail(VFPS_ERR_INVALID, "fused step: workgroup size");
     }
-    const void* fn = step_kernel_for(v, single, ceil_div(maxn, threads));
+    const int form = single ? STEP_SINGLE
+                            : (grid < nblocks ? STEP_PERSISTENT : STEP_PER_TILE);
+    const void* fn = step_kernel_for(v, form, ceil_div(maxn, threads));
     if (!fn) {
         return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
                                           "interpolation/stencil combination");
@@ -164,7 +139,7 @@ int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
 int step_occupancy(const StepVariant& v, uint32_t threads, uint32_t maxn,
                    size_t lds_bytes, int* per_cu)
 {
-    const void* fn = step_kernel_for(v, false, ceil_div(maxn, threads));
+    const void* fn = step_kernel_for(v, STEP_PERSISTENT, ceil_div(maxn, threads));
     if (!fn) {
         return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for this "
                                           "interpolation/stencil combination");
@@ -800,7 +775,7 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
     s->variant.direct = s->direct;
     s->variant.clamp = desc->rot_clamp != 0;
     if (s->fused) {
-        if (!step_kernel_for(s->variant)) {
+        if (!step_kernel_for(s->variant, STEP_SINGLE, 1)) {
             return fail(VFPS_ERR_UNSUPPORTED,
                         "vfps_step_create: fused step needs linear or cubic "
                         "interpolation (use fused = 0)");

@@ -55,8 +55,8 @@ struct StepVariant {
     bool direct, clamp;
 };
 
-const void* step_kernel_for(const StepVariant& v, bool single = false,
-                            uint32_t maxc = 1);
+// form: StepForm (step_kernel.hpp); maxc: columns per thread
+const void* step_kernel_for(const StepVariant& v, int form, uint32_t maxc);
 size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
                       uint32_t stride);
 int step_set_lds_limit(const StepVariant& v, size_t bytes);

@@ -131,6 +131,9 @@ typedef __attribute__((address_space(1))) uint32_t guint;
 constexpr int STEP_MAX_COLS = 4;
 // float4 loads one thread keeps in flight while filling the tile
 constexpr int STEP_LOAD_BATCH = 8;
+// float4 loads of the input rows per thread issued before the wake products
+// (one-mesh launches)
+constexpr int STEP_EARLY_LOADS = 2;
 // largest workgroup
 constexpr uint32_t STEP_MAX_THREADS = 512;
 // block-map entry of a padding workgroup
@@ -231,8 +234,11 @@ struct StepShared {
 // One row tile of one instance: phases A-H.  Every early return is uniform
 // over the workgroup.  MAXC: most mesh columns one thread handles
 // (ceil(n / threads) of every instance of the launch), which sizes the
-// per-column register tables.
-template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, int MAXC>
+// per-column register tables.  EARLY: float4 loads of the input rows per
+// thread held in registers through phase B (latency for registers: the
+// one-mesh launch uses them, a batch keeps its occupancy).
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, int MAXC,
+          int EARLY>
 __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                                           uint32_t tile, uint32_t logical,
                                           uint32_t parity, float* lds,
@@ -291,6 +297,22 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
     const bool csr = DIRECT && d.use_csr != 0;
 
     // ---- A: every global read -------------------------------------------
+    //  - the first EARLY float4 of the input rows per thread go out first
+    //    and stay in registers through phase B (stored to LDS after the
+    //    wake products), so B runs while the mesh is in flight
+    const bool vec_rows = (n & 3u) == 0u;
+    const size_t in_first = (size_t)(rbase + lr0) * n;
+    const uint32_t in_total = (uint32_t)(lr1 - lr0) * n;
+    const uint32_t in_total4 = in_total >> 2;
+    const gfloat4* in4 = (const gfloat4*)(in + in_first);
+    vec4f early[EARLY > 0 ? EARLY : 1];
+    if (vec_rows) {
+#pragma unroll
+        for (int b = 0; b < EARLY; b++) {
+            const uint32_t i = tid + (uint32_t)b * bs;
+            early[b] = in4[i < in_total4 ? i : in_total4 - 1u];
+        }
+    }
     // steps completed before this one (read by one thread, shared with the
     // workgroup after the barrier that ends phase A)
     if (tid == 0) {
@@ -360,42 +382,50 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             }
         }
     }
-    //  - input rows [rbase+lr0, rbase+lr1)
-    {
-        const size_t first = (size_t)(rbase + lr0) * n;
-        const uint32_t total = (uint32_t)(lr1 - lr0) * n;
+    //  - input rows [rbase+lr0, rbase+lr1) to LDS: the early registers, then
+    //    the rest in batches; the kick taps of the RF (and FFT wake) rows.
+    //    Without early loads this runs before the barrier that ends phase A
+    //    (after phase B it would keep the batch registers live through B).
+    auto fill_rows = [&]() {
         float* dst = bufA + (size_t)lr0 * n;
-        if ((n & 3u) == 0u) {
-            const gfloat4* in4 = (const gfloat4*)(in + first);
+        if (vec_rows) {
             vec4f* a4 = reinterpret_cast<vec4f*>(dst);
-            const uint32_t total4 = total >> 2;
-            for (uint32_t base = 0; base < total4; base += bs * STEP_LOAD_BATCH) {
+#pragma unroll
+            for (int b = 0; b < EARLY; b++) {
+                const uint32_t i = tid + (uint32_t)b * bs;
+                if (i < in_total4) a4[i] = early[b];
+            }
+            for (uint32_t base = (uint32_t)EARLY * bs; base < in_total4;
+                 base += bs * STEP_LOAD_BATCH) {
                 // unconditional loads (clamped index) keep v in registers
                 vec4f v[STEP_LOAD_BATCH];
 #pragma unroll
                 for (int b = 0; b < STEP_LOAD_BATCH; b++) {
                     const uint32_t i = base + tid + (uint32_t)b * bs;
-                    v[b] = in4[i < total4 ? i : total4 - 1u];
+                    v[b] = in4[i < in_total4 ? i : in_total4 - 1u];
                 }
 #pragma unroll
                 for (int b = 0; b < STEP_LOAD_BATCH; b++) {
                     const uint32_t i = base + tid + (uint32_t)b * bs;
-                    if (i < total4) a4[i] = v[b];
+                    if (i < in_total4) a4[i] = v[b];
                 }
             }
         } else {
-            for (uint32_t i = tid; i < total; i += bs) {
-                dst[i] = in[first + i];
+            for (uint32_t i = tid; i < in_total; i += bs) {
+                dst[i] = in[in_first + i];
             }
         }
-    }
-    if ((int)tid >= lr0 && (int)tid < lr1) {
-        if (SCHEME == 0) {
-            store_row_taps<IT>(offr[tid], n, tapr + tid * STEP_TAP_WORDS);
+        if ((int)tid >= lr0 && (int)tid < lr1) {
+            if (SCHEME == 0) {
+                store_row_taps<IT>(offr[tid], n, tapr + tid * STEP_TAP_WORDS);
+            }
+            if (!DIRECT && wake) {
+                store_row_taps<IT>(offw[tid], n, tapw + tid * STEP_TAP_WORDS);
+            }
         }
-        if (!DIRECT && wake) {
-            store_row_taps<IT>(offw[tid], n, tapw + tid * STEP_TAP_WORDS);
-        }
+    };
+    if constexpr (EARLY == 0) {
+        fill_rows();
     }
     __syncthreads();
     const uint32_t cnt = cnt_sh;
@@ -459,11 +489,14 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                 s00_sh = v;
             }
         }
-        __syncthreads();
-        if (csr && tid == 0) {
-            for (int o = 0; o < rows_out; o++) {
-                csr_q += csr_rows[o];
-            }
+    }
+    if constexpr (EARLY > 0) {
+        fill_rows();
+    }
+    __syncthreads();
+    if (DIRECT && csr && tid == 0) {
+        for (int o = 0; o < rows_out; o++) {
+            csr_q += csr_rows[o];
         }
     }
     VFPS_STAMP(2);
@@ -757,18 +790,26 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
 
 #undef VFPS_STAMP
 
+// Launch forms of the step kernel
+enum StepForm : int {
+    STEP_SINGLE = 0,      // one mesh, descriptor by value, one workgroup per tile
+    STEP_PER_TILE = 1,    // a batch, one workgroup per block-map entry
+    STEP_PERSISTENT = 2,  // a batch, a resident grid walking the block map
+};
+
 // The step of one launch.
-//  SINGLE: one mesh, its descriptor passed by value (the launch's own
+//  STEP_SINGLE: one mesh, its descriptor passed by value (the launch's own
 //  arguments, so no load of a descriptor or block map precedes the data
 //  loads), one workgroup per tile.
-//  Else a batch: blockmap[nmap] holds (instance << 16 | tile) per logical
-//  tile (STEP_NO_TILE: padding), nmap and the grid multiples of 8.
+//  Batches: blockmap[nmap] holds (instance << 16 | tile) per logical tile
+//  (STEP_NO_TILE: padding); nmap and the grid are multiples of 8.
 // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so the
 // map is cut into 8 contiguous runs, one per XCD (neighbouring tiles share
-// halo rows, which then hit in that XCD's L2).  The workgroups of an XCD
-// walk its run: one tile each when the grid holds the whole map, several
-// in a persistent launch (a smaller grid, sized to what stays resident).
-template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, bool SINGLE,
+// halo rows, which then hit in that XCD's L2).  STEP_PER_TILE: workgroup w
+// takes entry (w % 8) * (nmap / 8) + w / 8.  STEP_PERSISTENT: the
+// workgroups of an XCD walk its run with a stride of grid / 8 (the loop is
+// a form of its own: around the per-tile body it costs registers).
+template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, int FORM,
           int MAXC>
 __global__ void __launch_bounds__(STEP_MAX_THREADS)
 step_kernel(const InstDesc* __restrict__ descs,
@@ -778,14 +819,21 @@ step_kernel(const InstDesc* __restrict__ descs,
     extern __shared__ __align__(16) float lds[];
     __shared__ StepShared sh;
     const uint32_t G = gridDim.x;
-    if constexpr (SINGLE) {
+    if constexpr (FORM == STEP_SINGLE) {
         uint32_t logical = blockIdx.x;
         if ((G & 7u) == 0u) {
             logical = (blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
         }
-        step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC>(d0, 0u, logical,
-                                                         logical, parity, lds,
-                                                         sh);
+        step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, STEP_EARLY_LOADS>(
+            d0, 0u, logical, logical, parity, lds, sh);
+    } else if constexpr (FORM == STEP_PER_TILE) {
+        const uint32_t logical = (blockIdx.x & 7u) * (nmap >> 3) + (blockIdx.x >> 3);
+        const uint32_t m = blockmap[logical];
+        if (m == STEP_NO_TILE) {
+            return;  // padding of the map to a multiple of the XCD count
+        }
+        step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, 0>(
+            descs[m >> 16], m >> 16, m & 0xFFFFu, logical, parity, lds, sh);
     } else {
         const uint32_t per = nmap >> 3;
         const uint32_t wgs = G >> 3;
@@ -796,7 +844,7 @@ step_kernel(const InstDesc* __restrict__ descs,
             if (m == STEP_NO_TILE) {
                 continue;
             }
-            step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC>(
+            step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, 0>(
                 descs[m >> 16], m >> 16, m & 0xFFFFu, logical, parity, lds, sh);
             // the next tile reuses the LDS
             __syncthreads();
