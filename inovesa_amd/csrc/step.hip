@@ -449,6 +449,10 @@ static int step_upload_desc(vfps_step* s)
     d.hhi = hhi;
     s->hmax = hlo + hhi;
     s->stride = s->T + hlo + hhi;
+    // the fused kernel gives each buffer row (tile plus halo) one thread
+    VFPS_CHECK_ARG(!s->fused || s->stride <= s->threads,
+                   "fused step: tile rows plus halo rows exceed the workgroup "
+                   "size");
     d.T = s->T;
     d.stride = s->stride;
     VFPS_HIP(hipMemcpy(s->inst_d, &d, sizeof(InstDesc), hipMemcpyHostToDevice));

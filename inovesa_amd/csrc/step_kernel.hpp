@@ -299,7 +299,9 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
     // ---- A: every global read -------------------------------------------
     //  - the first EARLY float4 of the input rows per thread go out first
     //    and stay in registers through phase B (stored to LDS after the
-    //    wake products), so B runs while the mesh is in flight
+    //    wake products): the tables' stores below wait for them too (the
+    //    load counter drains in order), but the compiler keeps them here
+    //    instead of sinking them to their use after B
     const bool vec_rows = (n & 3u) == 0u;
     const size_t in_first = (size_t)(rbase + lr0) * n;
     const uint32_t in_total = (uint32_t)(lr1 - lr0) * n;
@@ -313,56 +315,82 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             early[b] = in4[i < in_total4 ? i : in_total4 - 1u];
         }
     }
-    // steps completed before this one (read by one thread, shared with the
-    // workgroup after the barrier that ends phase A)
-    if (tid == 0) {
-        cnt_sh = d.counter ? d.counter[parity] : 0u;
+    // The loads of the step counter, the previous profile, the kernel
+    // windows, row 0 and the row offsets go out before any of them is
+    // stored to LDS: the hardware counts outstanding loads in order, so a
+    // store that waits for its own load waits for every load issued before
+    // it (a load-store pair per table would cost a memory round trip each).
+    // Indices are clamped into the tables, the stores bounded.
+    //  - steps completed before this one (read by one thread, shared with
+    //    the workgroup after the barrier that ends phase A)
+    //  - previous profile and the kernel windows (direct wake): wake rows
+    //    [rbase+lr0, rbase+lr1) need g[r - x' + n - 1]: gwin[k] =
+    //    g[rbase + lr0 + k], k < (lr1 - lr0) + n - 1; CSR rows
+    //    [x0, x0+rows_out) need hwin[k] = h[x0 + k]
+    //  - row 0 of the input, for the out-of-range taps of the rotation in
+    //    tiles that do not hold it
+    //  - per-row kick offsets (their taps follow with the rows)
+    // window values per thread: gl, hl < n + stride <= (MAXC + 1) bs, as the
+    // host keeps stride <= threads (one thread per buffer row)
+    constexpr int WV = MAXC + 1;
+    const uint32_t g0 = (uint32_t)(rbase + lr0);
+    const uint32_t gl = (uint32_t)(lr1 - lr0) + n - 1;
+    const uint32_t hl = (uint32_t)rows_out + n - 1;
+    const bool need_row0 = DIRECT && SCHEME == 1 && rbase > 0 && wake;
+    const bool row_thread = (int)tid >= lr0 && (int)tid < lr1;
+    uint32_t cnt_v = 0;
+    if (tid == 0 && d.counter) {
+        cnt_v = d.counter[parity];
     }
-    //  - previous profile and the kernel windows (direct wake)
-    //    wake rows [rbase+lr0, rbase+lr1) need g[r - x' + n - 1]:
-    //    gwin[k] = g[rbase + lr0 + k], k < (lr1 - lr0) + n - 1;
-    //    CSR rows [x0, x0+rows_out) need hwin[k] = h[x0 + k]
+    float pv[MAXC];
+    double gv[WV], hv[WV];
+    float r0v[MAXC];
+    double g0v[MAXC];
     if (DIRECT) {
-        for (uint32_t x = tid; x < n; x += bs) {
-            pprev[x] = proj_in[x];
+#pragma unroll
+        for (int c = 0; c < MAXC; c++) {
+            const uint32_t x = tid + (uint32_t)c * bs;
+            pv[c] = proj_in[x < n ? x : n - 1u];
         }
         if (wake) {
-            const uint32_t g0 = (uint32_t)(rbase + lr0);
-            const uint32_t gl = (uint32_t)(lr1 - lr0) + n - 1;
-            for (uint32_t k = tid; k < gl; k += bs) {
-                gwin[k] = wake_g[g0 + k];
+#pragma unroll
+            for (int c = 0; c < WV; c++) {
+                const uint32_t k = tid + (uint32_t)c * bs;
+                gv[c] = wake_g[g0 + (k < gl ? k : gl - 1u)];
             }
         }
         if (csr) {
-            const uint32_t hl = (uint32_t)rows_out + n - 1;
-            for (uint32_t k = tid; k < hl; k += bs) {
-                hwin[k] = csr_h[(uint32_t)x0 + k];
+#pragma unroll
+            for (int c = 0; c < WV; c++) {
+                const uint32_t k = tid + (uint32_t)c * bs;
+                hv[c] = csr_h[(uint32_t)x0 + (k < hl ? k : hl - 1u)];
+            }
+        }
+        if (need_row0) {
+#pragma unroll
+            for (int c = 0; c < MAXC; c++) {
+                const uint32_t y = tid + (uint32_t)c * bs;
+                const uint32_t yy = y < n ? y : n - 1u;
+                r0v[c] = in[yy];
+                g0v[c] = wake_g[yy];
             }
         }
     }
-    //  - row 0 of the input, for the out-of-range taps of the rotation in
-    //    tiles that do not hold it
-    if (DIRECT && SCHEME == 1 && rbase > 0 && wake) {
-        float* row0 = lds + L.row0;
-        double* g0win = reinterpret_cast<double*>(lds + L.g0win);
-        for (uint32_t y = tid; y < n; y += bs) {
-            row0[y] = in[y];
-            g0win[y] = wake_g[y];
-        }
-    }
-    //  - per-row kick offsets; the RF taps (and FFT-wake taps) right away
-    if ((int)tid >= lr0 && (int)tid < lr1) {
+    float offr_v = 0.0f, offw_v = 0.0f;
+    if (row_thread) {
         const int r = rbase + (int)tid;
         if (SCHEME == 0) {
-            offr[tid] = rf_off[r];
+            offr_v = rf_off[r];
         }
         if (!DIRECT && wake) {
-            offw[tid] = wake_off[r];
+            offw_v = wake_off[r];
         }
     }
-    //  - per-column tables in registers
+    //  - per-column tables in registers (the drift taps are made from the
+    //    raw offsets once the stores below have waited for them)
     const int ncols = (int)((n + bs - 1) / bs);
     float col_ws[MAXC];
+    float col_doff[MAXC];
     uint32_t col_didx[MAXC][IT];   // drift taps of the column
     float col_dw[MAXC][IT];
     uint32_t col_fi[MAXC][IPFP > 0 ? IPFP : 1];
@@ -372,13 +400,66 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
         const uint32_t y = tid + (uint32_t)c * bs;
         if (c < ncols && y < n) {
             if (SCHEME == 0) {
-                kick_taps<IT>(drift_off[y], n, col_didx[c], col_dw[c]);
+                col_doff[c] = drift_off[y];
             }
             col_ws[c] = ws_g[y];
 #pragma unroll
             for (int k = 0; k < IPFP; k++) {
                 col_fi[c][k] = fp_i[(size_t)k * n + y];
                 col_fw[c][k] = fp_w[(size_t)k * n + y];
+            }
+        }
+    }
+    if (tid == 0) {
+        cnt_sh = cnt_v;
+    }
+    if (DIRECT) {
+#pragma unroll
+        for (int c = 0; c < MAXC; c++) {
+            const uint32_t x = tid + (uint32_t)c * bs;
+            if (x < n) pprev[x] = pv[c];
+        }
+        if (wake) {
+#pragma unroll
+            for (int c = 0; c < WV; c++) {
+                const uint32_t k = tid + (uint32_t)c * bs;
+                if (k < gl) gwin[k] = gv[c];
+            }
+        }
+        if (csr) {
+#pragma unroll
+            for (int c = 0; c < WV; c++) {
+                const uint32_t k = tid + (uint32_t)c * bs;
+                if (k < hl) hwin[k] = hv[c];
+            }
+        }
+        if (need_row0) {
+            float* row0 = lds + L.row0;
+            double* g0win = reinterpret_cast<double*>(lds + L.g0win);
+#pragma unroll
+            for (int c = 0; c < MAXC; c++) {
+                const uint32_t y = tid + (uint32_t)c * bs;
+                if (y < n) {
+                    row0[y] = r0v[c];
+                    g0win[y] = g0v[c];
+                }
+            }
+        }
+    }
+    if (row_thread) {
+        if (SCHEME == 0) {
+            offr[tid] = offr_v;
+        }
+        if (!DIRECT && wake) {
+            offw[tid] = offw_v;
+        }
+    }
+    if (SCHEME == 0) {
+#pragma unroll
+        for (int c = 0; c < MAXC; c++) {
+            const uint32_t y = tid + (uint32_t)c * bs;
+            if (c < ncols && y < n) {
+                kick_taps<IT>(col_doff[c], n, col_didx[c], col_dw[c]);
             }
         }
     }
