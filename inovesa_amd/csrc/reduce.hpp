@@ -52,4 +52,36 @@ __device__ __forceinline__ T block_sum(T v, T* part)
     return r;
 }
 
+// Two sums over the workgroup in one pass; `part` holds two entries per
+// wavefront (<= 8 wavefronts).  The per-wave and cross-wave orders are those
+// of block_sum.  Ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void block_sum2(T& a, T& b, T* part)
+{
+    const unsigned lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const unsigned nwaves = (blockDim.x + 63) / 64;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (lane == 0) {
+        part[wave] = a;
+        part[8 + wave] = b;
+    }
+    __syncthreads();
+    T ta = (threadIdx.x < nwaves) ? part[threadIdx.x] : T(0);
+    T tb = (threadIdx.x < nwaves) ? part[8 + threadIdx.x] : T(0);
+    if (wave == 0) {
+        // only wavefront 0 read the entries, before lane 0 writes
+        ta = wave_sum(ta);
+        tb = wave_sum(tb);
+        if (lane == 0) {
+            part[0] = ta;
+            part[8] = tb;
+        }
+    }
+    __syncthreads();
+    a = part[0];
+    b = part[8];
+    __syncthreads();
+}
+
 }  // namespace vfps_impl

@@ -229,21 +229,24 @@ struct StepShared {
     float s00;
     uint32_t slot;
     uint32_t cnt;
+    uint32_t cvalid;  // step whose CSR shares the previous launch left
 };
 
 // One row tile of one instance: phases A-H.  Every early return is uniform
 // over the workgroup.  MAXC: most mesh columns one thread handles
 // (ceil(n / threads) of every instance of the launch), which sizes the
-// per-column register tables.  EARLY: float4 loads of the input rows per
-// thread held in registers through phase B (latency for registers: the
-// one-mesh launch uses them, a batch keeps its occupancy).
+// per-column register tables.  ONE_MESH trades registers for latency, as
+// suits a launch of one mesh (a batch keeps its occupancy instead): the
+// first float4 loads of the input rows are held in registers through
+// phase B, and tile 0 loads what closes the step in phase A.
 template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, int MAXC,
-          int EARLY>
+          bool ONE_MESH>
 __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                                           uint32_t tile, uint32_t logical,
                                           uint32_t parity, float* lds,
                                           StepShared& sh)
 {
+    constexpr int EARLY = ONE_MESH ? STEP_EARLY_LOADS : 0;
     double* red = sh.red;
     float& s00_sh = sh.s00;
     uint32_t& slot_sh = sh.slot;
@@ -410,8 +413,25 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             }
         }
     }
+    //  - tile 0 closes the previous step: its CSR shares, the step they
+    //    belong to (phase H sums them)
+    const bool closer = ONE_MESH && DIRECT && csr && tile == 0;
+    uint32_t cvalid_v = 0;
+    double cpv[MAXC];
+    if (closer) {
+        if (tid == 0) {
+            cvalid_v = ((const guint*)d.csr_valid)[parity ^ 1u];
+        }
+        const gdouble* shares = (const gdouble*)d.csr_part + (size_t)(parity ^ 1u) * n;
+#pragma unroll
+        for (int c = 0; c < MAXC; c++) {
+            const uint32_t k = tid + (uint32_t)c * bs;
+            cpv[c] = shares[k < d.tiles ? k : d.tiles - 1u];
+        }
+    }
     if (tid == 0) {
         cnt_sh = cnt_v;
+        sh.cvalid = cvalid_v;
     }
     if (DIRECT) {
 #pragma unroll
@@ -826,21 +846,47 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             d.csr_part[(size_t)parity * n + tile] = csr_q;
         }
         if (tile == 0) {
-            double f = 0.0;
-            for (uint32_t x = tid; x < n; x += bs) {
-                f += (double)pprev[x] * (double)ws_g[x];
-            }
-            f = block_sum(f, red);
-            double p = 0.0;
-            // uniform over the workgroup: the sum below has barriers
-            const bool prev = csr && cnt >= 1 && d.csr_valid[parity ^ 1u] == cnt;
-            if (prev) {
-                double a = 0.0;
-                for (uint32_t k = tid; k < d.tiles; k += bs) {
-                    a += d.csr_part[(size_t)(parity ^ 1u) * n + k];
+            // the FP64 integral of the profile that entered the step (Simpson
+            // weights from the column registers: ws is the same on both
+            // axes) and the sum of the previous step's shares loaded in
+            // phase A, in one reduction
+            double f = 0.0, p = 0.0;
+            bool prev;
+            if constexpr (ONE_MESH) {
+#pragma unroll
+                for (int c = 0; c < MAXC; c++) {
+                    const uint32_t x = tid + (uint32_t)c * bs;
+                    if (c < ncols && x < n) {
+                        f += (double)pprev[x] * (double)col_ws[c];
+                    }
                 }
-                p = block_sum(a, red);
+                // uniform over the workgroup (read after phase A's barrier)
+                prev = csr && cnt >= 1 && sh.cvalid == cnt;
+                if (prev) {
+#pragma unroll
+                    for (int c = 0; c < MAXC; c++) {
+                        const uint32_t k = tid + (uint32_t)c * bs;
+                        if (k < d.tiles) {
+                            p += cpv[c];
+                        }
+                    }
+                }
+            } else {
+                // the same sums in the same order, read here
+                for (uint32_t x = tid; x < n; x += bs) {
+                    f += (double)pprev[x] * (double)ws_g[x];
+                }
+                prev = csr && cnt >= 1 &&
+                       ((const guint*)d.csr_valid)[parity ^ 1u] == cnt;
+                if (prev) {
+                    const gdouble* shares =
+                        (const gdouble*)d.csr_part + (size_t)(parity ^ 1u) * n;
+                    for (uint32_t k = tid; k < d.tiles; k += bs) {
+                        p += shares[k];
+                    }
+                }
             }
+            block_sum2(f, p, red);
             if (tid == 0) {
                 if (prev) {
                     *d.csr_power = (float)p;
@@ -905,7 +951,7 @@ step_kernel(const InstDesc* __restrict__ descs,
         if ((G & 7u) == 0u) {
             logical = (blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
         }
-        step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, STEP_EARLY_LOADS>(
+        step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, true>(
             d0, 0u, logical, logical, parity, lds, sh);
     } else if constexpr (FORM == STEP_PER_TILE) {
         const uint32_t logical = (blockIdx.x & 7u) * (nmap >> 3) + (blockIdx.x >> 3);
@@ -913,7 +959,7 @@ step_kernel(const InstDesc* __restrict__ descs,
         if (m == STEP_NO_TILE) {
             return;  // padding of the map to a multiple of the XCD count
         }
-        step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, 0>(
+        step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, false>(
             descs[m >> 16], m >> 16, m & 0xFFFFu, logical, parity, lds, sh);
     } else {
         const uint32_t per = nmap >> 3;
@@ -925,7 +971,7 @@ step_kernel(const InstDesc* __restrict__ descs,
             if (m == STEP_NO_TILE) {
                 continue;
             }
-            step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, 0>(
+            step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, false>(
                 descs[m >> 16], m >> 16, m & 0xFFFFu, logical, parity, lds, sh);
             // the next tile reuses the LDS
             __syncthreads();
