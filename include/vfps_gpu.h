@@ -362,10 +362,14 @@ VFPS_API int vfps_step_set_mesh(vfps_step* s, const float* mesh);
 VFPS_API int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
                               float* ms_maps, float* ms_step);
 /* one fused step with device timestamps at the phase boundaries of the
- * first workgroup of each kernel; ns_out (>= 16): [1..6] maps kernel
- * (loads, wake kick, RF kick, drift/rotation, FP, projection) and [9..13]
- * field kernel (loads, forward FFT, spectrum pass, inverse FFT, output),
- * nanoseconds since the kernel's first stamp; [15] the clock period */
+ * first workgroup of each kernel; ns_out (>= 16): [1..7] step kernel
+ * (loads, wake products, wake kick, RF kick, drift/rotation, FP,
+ * projection) and [9..13] field kernel (loads, forward FFT, spectrum pass,
+ * inverse FFT, output), nanoseconds since the kernel's first stamp; [15]
+ * the clock period; (nout >= 24) [16..21] step workgroups: first entry
+ * after the field kernel, entry spread, shortest, median and longest
+ * workgroup, last exit; (nout >= 24 + 2 tiles) [24 + 2t], [25 + 2t] entry
+ * and exit of tile t after the first entry */
 VFPS_API int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout);
 /* the configuration in use: out[0] fused, [1] wake method (1 FFT, 2 direct),
  * [2] tile rows, [3] threads, [4] LDS buffer rows, [5] LDS bytes (nout >= 6),

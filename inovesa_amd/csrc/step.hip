@@ -1061,6 +1061,16 @@ int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
             ns_out[19] = dur[dur.size() / 2];
             ns_out[20] = dur.back();
             ns_out[21] = (double)(x1 - e0) * ns_per_tick;
+            // [24 + 2 t], [25 + 2 t] (nout >= 24 + 2 tiles): entry and exit
+            // of tile t after the first entry
+            if (nout >= 24 + 2 * (size_t)nblk) {
+                for (uint32_t b = 0; b < nblk; b++) {
+                    const unsigned long long a = h[16 + 2 * b], z = h[17 + 2 * b];
+                    if (!a || !z) continue;
+                    ns_out[24 + 2 * b] = (double)(a - e0) * ns_per_tick;
+                    ns_out[25 + 2 * b] = (double)(z - e0) * ns_per_tick;
+                }
+            }
         }
     }
     return VFPS_OK;

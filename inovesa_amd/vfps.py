@@ -587,14 +587,19 @@ class Step:
              C.byref(c))
         return a.value, b.value, c.value
 
-    def phases(self):
+    def phases(self, tiles=False):
         """Device timestamps of one fused step (ns): the step kernel's phase
         ends in its first tile, the field kernel's phase ends (FFT wake
         only), and the step workgroups' (gap after the field kernel, entry
-        spread, shortest, median, longest, last exit)."""
-        out = np.zeros(24, np.float64)
-        call("vfps_step_phases", self.h, _p(out), 24)
-        return out[1:8].copy(), out[9:14].copy(), out[16:22].copy()
+        spread, shortest, median, longest, last exit); with tiles=True also
+        every tile's (entry, exit) after the first entry."""
+        ntiles = -(-self.n // self.config()["tile_rows"])
+        out = np.zeros(24 + 2 * ntiles, np.float64)
+        call("vfps_step_phases", self.h, _p(out), out.size)
+        r = (out[1:8].copy(), out[9:14].copy(), out[16:22].copy())
+        if tiles:
+            r += (out[24:].reshape(ntiles, 2).copy(),)
+        return r
 
     def config(self):
         """The configuration in use: fused, wake method (1 FFT, 2 direct),
