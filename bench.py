@@ -133,6 +133,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step from the host instead of "
+                         "replaying a hipGraph (for tools that do not "
+                         "follow graph launches)")
     ap.add_argument("--projection", choices=("wavefront", "ordered"),
                     default="ordered",
                     help="x projection: wavefront reduction, or the "
@@ -165,7 +169,8 @@ def main():
     # only this order keeps the per-step wake within 1e-5) or as a
     # wavefront reduction
     fast = 1 if args.projection == "wavefront" else 0
-    st = Step(dict(desc, fast_projection=fast), init, zw, zc)
+    st = Step(dict(desc, fast_projection=fast, no_graph=int(args.no_graph)),
+              init, zw, zc)
     st.record(args.warmup + args.steps, profiles=True)
     st.run(args.warmup)
     st.dev.sync()
@@ -191,7 +196,8 @@ def main():
                 "bytes_per_launch": bytes_per_launch}
 
     # the same workload with the other projection order
-    other = Step(dict(desc, fast_projection=1 - fast), init, zw, zc)
+    other = Step(dict(desc, fast_projection=1 - fast,
+                      no_graph=int(args.no_graph)), init, zw, zc)
     other.run(640)
     other.dev.sync()
     ko = min(args.steps, 12800)
@@ -205,7 +211,8 @@ def main():
     del other
 
     # the same workload with the FFT wake (field kernel + step kernel)
-    fft = Step(dict(desc, wake_method=1, fast_projection=fast), init, zw, zc)
+    fft = Step(dict(desc, wake_method=1, fast_projection=fast,
+                    no_graph=int(args.no_graph)), init, zw, zc)
     fft.run(640)
     fft.dev.sync()
     kf = min(args.steps, 6400)
@@ -232,6 +239,8 @@ def main():
                       "Ib_A": d["Ib"], "wake": "direct (FP64 Toeplitz)",
                       "x_projection": ("wavefront reduction" if fast else
                                        "sequential, reference order"),
+                      "launch": "host launches" if args.no_graph else
+                                "hipGraph of 64 steps",
                       "tile_rows": cfg["tile_rows"],
                       "threads": cfg["threads"]},
            "roofline": roofline, "fft_wake": fft_line,

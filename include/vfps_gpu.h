@@ -321,6 +321,8 @@ typedef struct {
                                  per step, wake and CSR power as FP64
                                  Toeplitz products of the profile */
     uint32_t threads;         /* fused step workgroup size (0: default) */
+    int32_t no_graph;         /* 1: vfps_step_run launches every step itself
+                                 instead of replaying a captured hipGraph */
 } vfps_step_desc;
 
 /* The FP64 Toeplitz kernels of the direct wake and CSR forms (host only,

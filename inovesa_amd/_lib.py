@@ -78,7 +78,7 @@ class StepDesc(C.Structure):
                 ("spacing_bins", _u32), ("wakescaling", _f32),
                 ("use_csr", _i32), ("fc", _f32), ("fused", _i32),
                 ("tile_rows", _u32), ("fast_projection", _i32),
-                ("wake_method", _u32), ("threads", _u32)]
+                ("wake_method", _u32), ("threads", _u32), ("no_graph", _i32)]
 
 
 _SIGS = {

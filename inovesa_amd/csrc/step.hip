@@ -852,7 +852,7 @@ int vfps_step_run(vfps_step* s, uint32_t nsteps)
         left--;
     }
     const uint32_t G = 64;
-    if (left >= G) {
+    if (left >= G && !s->desc.no_graph) {
         if (!s->graph || s->graph_steps != G) {
             if ((rc = step_capture(s, G)) != VFPS_OK) return rc;
         }

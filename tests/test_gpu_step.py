@@ -134,6 +134,21 @@ def test_graph_replay_deterministic(V):
     assert np.array_equal(a.mesh()[0], b.mesh()[0])
 
 
+def test_host_launches_equal_graph_replay(V):
+    """no_graph = 1 launches every step from the host: the same results as
+    the captured graph, bit for bit."""
+    desc, d, zw, zc, init = _setup(V, 0, 256, steps_per_ts=500)
+    a = V.Step(desc, init, zw, zc)
+    b = V.Step(dict(desc, no_graph=1), init, zw, zc)
+    a.record(140)
+    b.record(140)
+    a.run(131)
+    b.run(131)
+    for x, y in zip(a.mesh(), b.mesh()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.history()[0], b.history()[0])
+
+
 @pytest.mark.parametrize("scheme,n,kw", CASES[:2])
 def test_direct_and_fft_wake_agree(V, orc, scheme, n, kw):
     """The direct (Toeplitz) and FFT forms of the wake differ by rounding
