@@ -142,7 +142,7 @@ def main():
                     help="x projection: wavefront reduction, or the "
                          "reference's sequential order (bit-exact)")
     ap.add_argument("--batch-steps", type=int, default=2000)
-    ap.add_argument("--scan-targets", type=int, default=48)
+    ap.add_argument("--scan-targets", type=int, default=128)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB and exit")
     args = ap.parse_args()

@@ -409,6 +409,11 @@ VFPS_API int vfps_batch_get_csr_histories(vfps_batch* b, uint32_t first,
 /* instances, launch groups and mesh cells per step */
 VFPS_API int vfps_batch_info(vfps_batch* b, uint32_t* ninst,
                              uint32_t* ngroups, uint64_t* cells);
+/* vfps_batch_retile for every instance of mesh size n at once (one
+ * rebuild of the launches); on failure the previous tiling stays */
+VFPS_API int vfps_batch_retile_size(vfps_batch* b, uint32_t n,
+                                    uint32_t tile_rows, uint32_t threads,
+                                    int persistent);
 /* the launches of one step, cap rows of 4: workgroup size, tiles in the
  * block map (with padding), workgroups launched (fewer than the tiles:
  * persistent), dynamic LDS bytes; *ngroups = number of launches */

@@ -120,11 +120,14 @@ int build_group(vfps_batch* b, Group& g)
     VFPS_HIP(hipMalloc(&g.blockmap_d, sizeof(uint32_t) * map.size()));
     VFPS_HIP(hipMemcpy(g.blockmap_d, map.data(), sizeof(uint32_t) * map.size(),
                        hipMemcpyHostToDevice));
-    VFPS_HIP(hipMalloc(&g.descs_d, sizeof(InstDesc) * g.members.size()));
+    // the members' descriptors (host copies), one upload
+    std::vector<InstDesc> descs(g.members.size());
     for (size_t k = 0; k < g.members.size(); k++) {
-        VFPS_HIP(hipMemcpy(g.descs_d + k, b->members[g.members[k]]->inst_d,
-                           sizeof(InstDesc), hipMemcpyDeviceToDevice));
+        descs[k] = b->members[g.members[k]]->inst_h;
     }
+    VFPS_HIP(hipMalloc(&g.descs_d, sizeof(InstDesc) * descs.size()));
+    VFPS_HIP(hipMemcpy(g.descs_d, descs.data(), sizeof(InstDesc) * descs.size(),
+                       hipMemcpyHostToDevice));
     int rc = step_set_lds_limit(g.variant, g.lds);
     if (rc != VFPS_OK) return rc;
     if (g.persist) {
@@ -467,6 +470,45 @@ int vfps_batch_retile(vfps_batch* b, uint32_t inst, uint32_t tile_rows,
         // keep the batch runnable with the previous tiling
         s->persist = p0;
         step_retile(s, T0, thr0);
+        build_groups(b);
+        return rc;
+    }
+    return VFPS_OK;
+    VFPS_TRY_END
+}
+
+int vfps_batch_retile_size(vfps_batch* b, uint32_t n, uint32_t tile_rows,
+                           uint32_t threads, int persistent)
+{
+    VFPS_CHECK_ARG(b, "vfps_batch_retile_size: NULL batch");
+    VFPS_CHECK_ARG(tile_rows >= 1 && threads >= 64 && threads % 64 == 0 &&
+                       threads <= STEP_MAX_THREADS,
+                   "vfps_batch_retile_size: tile rows or workgroup size");
+    VFPS_TRY_BEGIN
+    VFPS_HIP(hipStreamSynchronize(b->dev->stream));
+    sync_parity(b);
+    struct Old {
+        vfps_step* s;
+        uint32_t T, thr;
+        bool p;
+    };
+    std::vector<Old> old;
+    int rc = VFPS_OK;
+    for (auto* s : b->members) {
+        if (s->desc.n != n) continue;
+        old.push_back({s, s->T, s->threads, s->persist});
+        s->persist = persistent != 0;
+        if ((rc = step_retile(s, tile_rows, threads)) != VFPS_OK) break;
+    }
+    if (rc == VFPS_OK) {
+        rc = build_groups(b);
+    }
+    if (rc != VFPS_OK) {
+        // keep the batch runnable with the previous tiling
+        for (auto& o : old) {
+            o.s->persist = o.p;
+            step_retile(o.s, o.T, o.thr);
+        }
         build_groups(b);
         return rc;
     }

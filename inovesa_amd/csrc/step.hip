@@ -18,7 +18,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <map>
 #include <memory>
+#include <mutex>
 
 #include "common.hpp"
 #include "field_body.hpp"
@@ -71,6 +73,11 @@ size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
 
 int step_set_lds_limit(const StepVariant& v, size_t bytes)
 {
+    // the largest limit already set per kernel (raising it is a runtime
+    // call; retiling the instances of a batch asks for it many times)
+    static std::mutex mu;
+    static std::map<const void*, size_t> set;
+    std::lock_guard<std::mutex> lock(mu);
     for (int form : {STEP_SINGLE, STEP_PER_TILE, STEP_PERSISTENT}) {
         for (uint32_t maxc : {1u, 2u, (uint32_t)STEP_MAX_COLS}) {
             const void* fn = step_kernel_for(v, form, maxc);
@@ -82,9 +89,10 @@ int step_set_lds_limit(const StepVariant& v, size_t bytes)
                                                   "this interpolation/stencil "
                                                   "combination");
             }
-            if (bytes > 65536) {
+            if (bytes > 65536 && bytes > set[fn]) {
                 VFPS_HIP(hipFuncSetAttribute(
                     fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+                set[fn] = bytes;
             }
         }
     }
@@ -436,15 +444,19 @@ static int step_upload_desc(vfps_step* s)
     d.counter = s->counter;
     d.err = s->err;
     d.seq_proj = s->desc.fast_projection ? 0u : 1u;
-    uint32_t hlo = 0, hhi = 0;
-    if (s->desc.scheme == 0) {
-        std::vector<float> off(t1->n);
-        VFPS_HIP(hipMemcpy(off.data(), s->drm->offset, sizeof(float) * t1->n,
-                           hipMemcpyDeviceToHost));
-        kick_halo(off, t1->n, s->desc.interp, &hlo, &hhi);
-    } else {
-        rotation_halo(d.rot, t1->n, s->desc.interp, &hlo, &hhi);
+    // the halo depends on the maps only: computed once
+    if (!s->halo_valid) {
+        if (s->desc.scheme == 0) {
+            std::vector<float> off(t1->n);
+            VFPS_HIP(hipMemcpy(off.data(), s->drm->offset, sizeof(float) * t1->n,
+                               hipMemcpyDeviceToHost));
+            kick_halo(off, t1->n, s->desc.interp, &s->halo_lo, &s->halo_hi);
+        } else {
+            rotation_halo(d.rot, t1->n, s->desc.interp, &s->halo_lo, &s->halo_hi);
+        }
+        s->halo_valid = true;
     }
+    const uint32_t hlo = s->halo_lo, hhi = s->halo_hi;
     d.hlo = hlo;
     d.hhi = hhi;
     s->hmax = hlo + hhi;
@@ -455,7 +467,6 @@ static int step_upload_desc(vfps_step* s)
                    "size");
     d.T = s->T;
     d.stride = s->stride;
-    VFPS_HIP(hipMemcpy(s->inst_d, &d, sizeof(InstDesc), hipMemcpyHostToDevice));
     s->inst_h = d;
 
     FieldParams p{};
@@ -491,8 +502,23 @@ static int step_upload_desc(vfps_step* s)
             p.tw_n = s->csr_tw + s->M;
         }
     }
-    VFPS_HIP(hipMemcpy(s->fparams_d, &p, sizeof(FieldParams),
-                       hipMemcpyHostToDevice));
+    s->fparams_h = p;
+    // uploaded before the next launch that reads them (step_flush_desc), so
+    // retiling the instances of a batch costs no device copies
+    s->desc_dirty = true;
+    return VFPS_OK;
+}
+
+// the device copies of the descriptors, if the host copies changed
+static int step_flush_desc(vfps_step* s)
+{
+    if (s->desc_dirty) {
+        VFPS_HIP(hipMemcpy(s->inst_d, &s->inst_h, sizeof(InstDesc),
+                           hipMemcpyHostToDevice));
+        VFPS_HIP(hipMemcpy(s->fparams_d, &s->fparams_h, sizeof(FieldParams),
+                           hipMemcpyHostToDevice));
+        s->desc_dirty = false;
+    }
     return VFPS_OK;
 }
 
@@ -844,7 +870,8 @@ int vfps_step_run(vfps_step* s, uint32_t nsteps)
     VFPS_CHECK_ARG(s, "vfps_step_run: NULL step");
     hipStream_t st = s->dev->stream;
     uint32_t left = nsteps;
-    int rc;
+    int rc = step_flush_desc(s);
+    if (rc != VFPS_OK) return rc;
     // bring the fused ping-pong to parity 0 before replaying the graph
     if (s->fused && (s->parity & 1u) && left > 0) {
         if ((rc = step_launch_one(s, 1)) != VFPS_OK) return rc;
@@ -996,6 +1023,7 @@ int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
     VFPS_CHECK_ARG(s->fused, "vfps_step_phases: needs the fused step");
     hipStream_t st = s->dev->stream;
     VFPS_HIP(hipStreamSynchronize(st));
+    if (int rc0 = step_flush_desc(s); rc0 != VFPS_OK) return rc0;
     const uint32_t nblk = s->tiles();
     const size_t nstamp = 16 + 2 * (size_t)nblk;
     unsigned long long* buf = nullptr;
@@ -1082,6 +1110,7 @@ int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
     VFPS_CHECK_ARG(s && nsteps > 0, "vfps_step_timing: bad argument");
     VFPS_CHECK_ARG(s->fused, "vfps_step_timing: per-kernel timing needs the "
                              "fused step");
+    if (int rc0 = step_flush_desc(s); rc0 != VFPS_OK) return rc0;
     hipStream_t st = s->dev->stream;
     hipEvent_t e0, e1, e2;
     VFPS_HIP(hipEventCreate(&e0));

@@ -128,6 +128,10 @@ struct vfps_step {
     vfps_impl::InstDesc* inst_d = nullptr;
     vfps_impl::InstDesc inst_h{};     // host copy, passed by value at launch
     vfps_impl::FieldParams* fparams_d = nullptr;
+    vfps_impl::FieldParams fparams_h{};
+    bool desc_dirty = true;           // inst_h/fparams_h not yet on the device
+    bool halo_valid = false;          // halo rows of the drift/rotation
+    uint32_t halo_lo = 0, halo_hi = 0;
     vfps_impl::StepVariant variant{};
     uint32_t T = 1, threads = 256, stride = 1, hmax = 0, M = 0;
     bool persist = false;  // in a batch: persistent launch (grid < tiles)

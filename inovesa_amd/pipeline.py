@@ -33,7 +33,10 @@ def scan_targets(count, seed=11):
 
 
 def compare(model, targets, setup, rel_tol=0.02):
-    """Dense and surrogate scans of the same targets: a JSON-able dict."""
+    """Dense and surrogate scans of the same targets: a JSON-able dict.
+    Each scan tunes the tiling of every batch-size class it meets once, and
+    that tuning counts as its GPU time (dense_tune_s, surrogate_tune_s are
+    the parts of the GPU times spent tuning)."""
     t0 = time.perf_counter()
     dres, drun = dense_scan(targets, setup, rel_tol=rel_tol)
     t1 = time.perf_counter()
@@ -52,6 +55,10 @@ def compare(model, targets, setup, rel_tol=0.02):
     return dict(
         targets=len(targets), rel_tol=rel_tol,
         dense_gpu_s=drun.gpu_seconds, surrogate_gpu_s=srun.gpu_seconds,
+        dense_tune_s=drun.tune_seconds, surrogate_tune_s=srun.tune_seconds,
+        run_time_ratio=(srun.gpu_seconds - srun.tune_seconds)
+        / max(drun.gpu_seconds - drun.tune_seconds, 1e-12),
+        dense_rounds=drun.rounds, surrogate_rounds=srun.rounds,
         dense_wall_s=t1 - t0, surrogate_wall_s=t2 - t1,
         dense_runs=drun.runs, surrogate_runs=srun.runs,
         gpu_time_ratio=srun.gpu_seconds / max(drun.gpu_seconds, 1e-12),
@@ -71,6 +78,10 @@ def main(argv=None):
     ap.add_argument("--out", default="gpurun_out/surrogate")
     ap.add_argument("--model", default=None,
                     help="load a trained model instead of training")
+    ap.add_argument("--data", default=None,
+                    help="train on the runs of a training_data.npz written "
+                         "by an earlier call (labels recomputed from their "
+                         "fluctuations with this setup's detector)")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
     setup = ScanSetup()
@@ -79,8 +90,15 @@ def main(argv=None):
     if args.model:
         model = Surrogate.load(args.model)
     else:
-        pts, spec, lab, fl, gpu_s = generate(args.runs, setup, chunk=args.chunk,
-                                             log=log, focus=args.focus)
+        if args.data:
+            z = np.load(args.data)
+            pts, spec, fl = z["points"], z["spectra"], z["fluct"]
+            lab = ((~np.isfinite(fl)) | (fl > setup.tol)).astype(np.float32)
+            gpu_s = 0.0
+        else:
+            pts, spec, lab, fl, gpu_s = generate(args.runs, setup,
+                                                 chunk=args.chunk, log=log,
+                                                 focus=args.focus)
         np.savez_compressed(os.path.join(args.out, "training_data.npz"),
                             points=pts, spectra=spec, labels=lab, fluct=fl)
         log(f"data: {len(lab)} runs, {int(lab.sum())} unstable, "

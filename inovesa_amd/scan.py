@@ -7,9 +7,11 @@ seeded Gaussian, and classifies each run with a detector of the bursting
 damping time is bracketed by the stable and unstable currents.
 
 Detector: the relative fluctuation std(P)/mean(P) of the CSR power P over
-the last `window` synchrotron periods of the run.  Below the threshold the
-damped bunch settles and P is constant to a few 1e-4; above it the bunch
-bursts and P fluctuates by 1e-2 and more; `tol` (5e-3) separates the two.
+the last `window` synchrotron periods of the run, against `tol`.  With the
+kick-drift maps a damped bunch below the threshold holds P constant to
+1e-4 and less, and a bursting one fluctuates by 3e-3 and more: tol = 1e-3.
+With the rotation maps the two levels are a few 1e-4 and 1e-2 and more:
+tol = 5e-3.
 
 Dense scan (the plain method): rounds of `points` currents spaced
 geometrically over the bracket, starting from [lo, hi] times the
@@ -30,7 +32,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import vfps as V
-from .workloads import derive, initial_mesh
+from .workloads import derive, initial_mesh, scan_instance
 
 C_LIGHT = 2.99792458e8
 
@@ -41,8 +43,14 @@ class ScanSetup:
     steps_per_ts: int = 500      # time steps per synchrotron period
     periods: int = 30            # synchrotron periods per run
     window: int = 5              # periods at the end seen by the detector
-    tol: float = 5e-3            # relative CSR fluctuation marking bursting
+    tol: float = None            # relative CSR fluctuation marking bursting
+                                 # (None: 1e-3 kick-drift, 5e-3 rotation)
     seed: int = 1                # initial-noise seed (the same for all runs)
+    scheme: str = "kick_drift"   # maps of every run (workloads.scan_instance)
+
+    def __post_init__(self):
+        if self.tol is None:
+            self.tol = 1e-3 if self.scheme == "kick_drift" else 5e-3
 
     @property
     def steps(self):
@@ -57,23 +65,10 @@ def scaling_law(gap, setup):
 
 def instance(factor, gap, damping_steps, setup):
     """One run: current factor * scaling-law threshold, parallel-plates CSR
-    with the given full gap (m), damping time in time steps; rotation
-    scheme with the cubic clamped RotationMap (desc, derived, init)."""
-    base = derive(grid_size=setup.n, steps_per_ts=setup.steps_per_ts,
-                  vacuum_gap=gap)
-    d = derive(grid_size=setup.n, steps_per_ts=setup.steps_per_ts,
-               vacuum_gap=gap, damping_time=damping_steps * base["dt"],
-               bunch_current=factor * base["Ith"])
-    desc = dict(n=setup.n, qmin=d["qmin"], qmax=d["qmax"], pmin=d["pmin"],
-                pmax=d["pmax"], qscale=d["qscale"], pscale=d["pscale"],
-                scheme=1, interp=4, rot_clamp=1, angle=d["angle"],
-                rf_linear=1, rf_angle=d["angle"],
-                revolutionpart=d["revolutionpart"], V_RF=d["V_eff"],
-                f_RF=d["f_RF"], V0=d["V0"], slip=d["slip"],
-                E0=d["ring"]["beam_energy"], use_fp=1, fptype=3, e1=d["e1"],
-                deriv=4, use_wake=1, spacing_bins=d["spacing_bins"],
-                wakescaling=d["wakescaling"], use_csr=1,
-                fc=d["ring"]["cutoff_frequency"])
+    with the given full gap (m), damping time in time steps, the maps of
+    setup.scheme (desc, derived, init)."""
+    desc, d = scan_instance(setup.n, factor, gap, damping_steps,
+                            setup.steps_per_ts, setup.scheme)
     return desc, d, initial_mesh(setup.n, seed=setup.seed)
 
 
@@ -97,20 +92,37 @@ class RunResult:
     spectrum: np.ndarray = field(repr=False, default=None)
 
 
+def batch_class(count):
+    """Batch-size class of `count` runs for the tiling cache: the next power
+    of two, at least 16."""
+    c = 16
+    while c < count:
+        c *= 2
+    return c
+
+
 class Runner:
     """Runs sets of (factor, gap, damping) points as one batch and keeps
-    the GPU wall time and the number of VFP runs it spent."""
+    the GPU wall time and the number of VFP runs it spent.  The first batch
+    of each mesh size is autotuned (its tuning time counts as GPU time);
+    later batches of that size reuse the tiling."""
 
     def __init__(self, setup=None, dev=None, spectra=False, snapshots=5,
-                 tune=True):
+                 tune=True, tiles=None):
         self.setup = setup or ScanSetup()
         self.dev = dev
         self.spectra = spectra
         self.snapshots = snapshots
         self.tune = tune
         self.gpu_seconds = 0.0
+        self.tune_seconds = 0.0      # part of gpu_seconds spent tuning
         self.runs = 0
         self.tuning = None
+        # (mesh size, batch class) -> (T, threads, persistent); a dict
+        # shared between runners makes the tiling a one-time cost (like
+        # FFTW wisdom)
+        self.tiles = {} if tiles is None else tiles
+        self.rounds = []             # (runs, GPU seconds) of every batch
 
     def run(self, points):
         s = self.setup
@@ -128,11 +140,23 @@ class Runner:
             ds.append(d)
         b = V.Batch(descs, inits, zws, zcs, dev=self.dev)
         if self.tune:
-            # tile shape per mesh size (results do not depend on it); the
-            # tuning steps count as GPU time of the scan
+            # tile shape per mesh size and batch-size class (a few runs want
+            # short tiles for latency, many runs tall ones for throughput;
+            # results do not depend on it); the tuning steps count as GPU
+            # time of the scan
             t0 = time.perf_counter()
-            self.tuning = b.autotune(steps=8)
-            self.gpu_seconds += time.perf_counter() - t0
+            key = (s.n, batch_class(len(points)))
+            if key in self.tiles:
+                b.retile_size(s.n, *self.tiles[key])
+            else:
+                self.tuning = b.autotune(steps=8)
+                for r in self.tuning:
+                    if r["chosen"] and r["n"] == s.n:
+                        self.tiles[key] = (r["T"], r["threads"],
+                                           r["persistent"])
+            dt = time.perf_counter() - t0
+            self.gpu_seconds += dt
+            self.tune_seconds += dt
         w = s.window * s.steps_per_ts
         b.record(s.steps)
         spectra = [np.zeros(64) for _ in points] if self.spectra else None
@@ -147,8 +171,10 @@ class Runner:
         else:
             b.run(s.steps)
         b.dev.sync()
-        self.gpu_seconds += time.perf_counter() - t0
+        dt_run = time.perf_counter() - t0
+        self.gpu_seconds += dt_run
         self.runs += len(points)
+        self.rounds.append((len(points), dt_run))
         hist = b.csr_histories(0, s.steps)
         profs = b.projections()
         out = []
@@ -225,6 +251,30 @@ def _refine(runner, results, points=16, rel_tol=0.02, max_rounds=8):
     return results
 
 
+def tune_tiles(setup, count, gap=0.04, damping=2500.0):
+    """The tiling of setup's mesh size, autotuned once on a batch of `count`
+    runs (currents spread over 0.5-2x the scaling law): a dict to pass to
+    Runner(tiles=...), and the GPU seconds the tuning took."""
+    tiles = {}
+    runner = Runner(setup, tiles=tiles)
+    pts = [(float(f), gap, damping) for f in np.geomspace(0.5, 2.0, count)]
+    descs, inits, zs = [], [], []
+    for f, g, dmp in pts:
+        desc, d, init = instance(f, g, dmp, setup)
+        zw = V.ParallelPlatesCSR(d["padded_bins"], np.float32(d["f0"]),
+                                 d["fmax"], g, dev=runner.dev)
+        descs.append(desc)
+        inits.append(init)
+        zs.append(zw)
+    b = V.Batch(descs, inits, zs, zs, dev=runner.dev)
+    t0 = time.perf_counter()
+    for r in b.autotune(steps=8):
+        if r["chosen"]:
+            tiles[(r["n"], batch_class(count))] = (r["T"], r["threads"],
+                                                   r["persistent"])
+    return tiles, time.perf_counter() - t0
+
+
 def dense_scan(targets, setup=None, lo=0.3, hi=3.0, points=16, rel_tol=0.02,
                runner=None):
     """Plain threshold scan of every (gap, damping) in targets; returns
@@ -236,7 +286,7 @@ def dense_scan(targets, setup=None, lo=0.3, hi=3.0, points=16, rel_tol=0.02,
 
 
 def surrogate_scan(model, targets, setup=None, lo=0.3, hi=3.0, points=16,
-                   rel_tol=0.02, runner=None, window=0.04, near=None):
+                   rel_tol=0.02, runner=None, window=0.03, near=None):
     """Threshold scan steered by the surrogate: `near` VFP runs per target
     spaced geometrically over [f*/(1+window), f*(1+window)] around the
     predicted threshold f* (by default as many as bring the bracket under

@@ -25,7 +25,7 @@ DIMS = (3, 256, 256, 256, 65)
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                              "data", "surrogate.npz")
 FACTOR_RANGE = (0.3, 3.0)       # current / scaling-law threshold
-FOCUS_RANGE = (0.8, 1.6)        # where the thresholds of the ranges below lie
+FOCUS_RANGE = (0.75, 1.8)       # where the thresholds of the ranges below lie
 GAP_RANGE = (0.016, 0.064)      # full chamber gap (m)
 DAMPING_RANGE = (1000.0, 5000.0)  # damping time (time steps)
 

@@ -706,6 +706,11 @@ class Batch:
         return [dict(threads=int(r[0]), tiles=int(r[1]), workgroups=int(r[2]),
                      lds_bytes=int(r[3])) for r in out[:min(ng.value, cap)]]
 
+    def retile_size(self, n, tile_rows, threads, persistent=False):
+        """Set the tiling of every instance of mesh size n."""
+        call("vfps_batch_retile_size", self.h, n, tile_rows, threads,
+             1 if persistent else 0)
+
     def retile(self, i, tile_rows, threads, persistent=False):
         """Set instance i's tiling by hand (results do not depend on it)."""
         call("vfps_batch_retile", self.h, i, tile_rows, threads,

@@ -124,19 +124,46 @@ def impedances(d, dev=None, resistive_wall=None):
     return zw, zc
 
 
+def scan_instance(n, factor, gap, damping_steps, steps_per_ts, scheme="kick_drift"):
+    """One run of a scan: current factor x the scaling-law threshold,
+    parallel-plates CSR with full gap `gap` (m), damping time damping_steps
+    time steps.  scheme "kick_drift": the maps of the plain 512^2 workload
+    (wake kick, sinusoidal RF kick, linear drift, cubic Fokker-Planck;
+    src/main.cpp:1034-1041); "rotation": wake kick, cubic clamped
+    RotationMap, Fokker-Planck.  Returns (desc, derived)."""
+    base = derive(grid_size=n, steps_per_ts=steps_per_ts, vacuum_gap=gap)
+    d = derive(grid_size=n, steps_per_ts=steps_per_ts, vacuum_gap=gap,
+               damping_time=damping_steps * base["dt"],
+               bunch_current=factor * base["Ith"])
+    kd = scheme == "kick_drift"
+    if not kd and scheme != "rotation":
+        raise ValueError(f"scheme must be kick_drift or rotation: {scheme}")
+    desc = dict(n=n, qmin=d["qmin"], qmax=d["qmax"], pmin=d["pmin"],
+                pmax=d["pmax"], qscale=d["qscale"], pscale=d["pscale"],
+                scheme=0 if kd else 1, interp=4, rot_clamp=0 if kd else 1,
+                angle=d["angle"], rf_linear=0 if kd else 1,
+                rf_angle=d["angle"], revolutionpart=d["revolutionpart"],
+                V_RF=d["V_eff"], f_RF=d["f_RF"], V0=d["V0"], slip=d["slip"],
+                E0=d["ring"]["beam_energy"], use_fp=1, fptype=3, e1=d["e1"],
+                deriv=4, use_wake=1, spacing_bins=d["spacing_bins"],
+                wakescaling=d["wakescaling"], use_csr=1,
+                fc=d["ring"]["cutoff_frequency"])
+    return desc, d
+
+
 def batched_scan_config(sizes=(128, 256, 384, 512), per_size=16, seed=3,
                         current_range=(0.3, 3.0), gap_range=(0.016, 0.064),
-                        steps_per_ts=1000, damping_steps=1.0e4):
+                        steps_per_ts=2000, damping_steps=1.0e4,
+                        scheme="kick_drift"):
     """Independent instances of uneven mesh sizes for a threshold scan:
     per_size instances of each size (sizes interleaved), currents
     log-spaced over current_range times each instance's scaling-law
     threshold, parallel-plates gaps drawn uniformly from gap_range (m) with
-    the given seed, and the rotation scheme (cubic clamped RotationMap,
-    FokkerPlanckMap with damping time damping_steps steps, wake kick) with
-    each instance's own parallel-plates CSR impedance (the default damping
-    keeps the diffusion number below 1/2 up to n = 512; see
-    rotation_config).  Returns a list of (desc, derived, init) per
-    instance."""
+    the given seed, each instance with its own parallel-plates CSR
+    impedance, the maps of `scheme` (see scan_instance) and a damping time
+    of damping_steps steps (the default keeps the Fokker-Planck diffusion
+    number below 1/2 up to n = 512; see rotation_config).  Returns a list
+    of (desc, derived, init) per instance."""
     k = len(sizes) * per_size
     rng = np.random.Generator(np.random.Philox(seed))
     gaps = rng.uniform(gap_range[0], gap_range[1], k)
@@ -144,22 +171,7 @@ def batched_scan_config(sizes=(128, 256, 384, 512), per_size=16, seed=3,
     out = []
     for i in range(k):
         n = sizes[i % len(sizes)]
-        base = derive(grid_size=n, steps_per_ts=steps_per_ts,
-                      vacuum_gap=float(gaps[i]))
-        d = derive(grid_size=n, steps_per_ts=steps_per_ts,
-                   vacuum_gap=float(gaps[i]),
-                   damping_time=damping_steps * base["dt"],
-                   bunch_current=float(factors[i]) * base["Ith"])
-        desc = dict(n=n, qmin=d["qmin"], qmax=d["qmax"], pmin=d["pmin"],
-                    pmax=d["pmax"], qscale=d["qscale"], pscale=d["pscale"],
-                    scheme=1, interp=4, rot_clamp=1, angle=d["angle"],
-                    rf_linear=1, rf_angle=d["angle"],
-                    revolutionpart=d["revolutionpart"], V_RF=d["V_eff"],
-                    f_RF=d["f_RF"], V0=d["V0"], slip=d["slip"],
-                    E0=d["ring"]["beam_energy"], use_fp=1, fptype=3,
-                    e1=d["e1"], deriv=4, use_wake=1,
-                    spacing_bins=d["spacing_bins"],
-                    wakescaling=d["wakescaling"], use_csr=1,
-                    fc=d["ring"]["cutoff_frequency"])
+        desc, d = scan_instance(n, float(factors[i]), float(gaps[i]),
+                                damping_steps, steps_per_ts, scheme)
         out.append((desc, d, initial_mesh(n, seed=seed * 1000 + i)))
     return out

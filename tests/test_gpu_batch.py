@@ -100,6 +100,10 @@ def test_retile_rejects_and_keeps_tiling(V):
     with pytest.raises(V.VfpsInvalid):
         b.retile(0, 1, 100)
     assert b.config(0) == before
+    b.retile_size(128, 2, 256, persistent=True)
+    for k in range(len(inst)):
+        cfg = b.config(k)
+        assert (cfg["tile_rows"], cfg["threads"], cfg["persistent"]) == (2, 256, 1)
     b.run(3)
 
 
