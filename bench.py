@@ -11,14 +11,21 @@ Prints one JSON line.  value = mesh-cell updates per second
 (n^2 * steps / wall time of the timed steps, state resident on the GPU).
 The x projection is summed in the reference's sequential order, bit for
 bit (--projection wavefront: as a wavefront reduction; other_projection
-reports the order not chosen).  roofline: the step kernel (the whole step in one
-launch; 8 bytes of mesh traffic per cell) timed with HIP events on its
-stream.  fft_wake: the same workload with the wake and CSR power from the
-FFT field kernel.  batched:
-64 independent meshes of 128^2-512^2 in one launch per step.  scan: the
-threshold scan with and without the trained network.  cpu_baseline: the
-oracle (the FP32 C restatement of the reference CPU path) on one host
-core, on a bounded sample of the same workload.
+reports the order not chosen).
+
+Further keys of the line:
+  roofline      the step kernel (the whole step in one launch; 8 bytes of
+                mesh traffic per cell), timed with HIP events on its stream;
+  fft_wake      the same workload with the wake and CSR power from the FFT
+                field kernel;
+  batched       64 independent meshes of 128^2-512^2 (kick-drift maps), one
+                launch per step and size class;
+  production    16 meshes of 1024^2 with the large production maps and
+                impedance;
+  scan          the threshold scan with and without the trained network;
+  cpu_baseline  the oracle (the FP32 C restatement of the reference CPU
+                path) on one host core, on a bounded sample of the same
+                workload.
 """
 import argparse
 import json
@@ -88,6 +95,46 @@ def batched_throughput(steps, log):
             "steps": steps, "ms_per_step": dt * 1e3 / steps,
             "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
             "tiles_chosen": {str(k): v for k, v in chosen.items()}}
+
+
+def production_throughput(steps, log):
+    """The large production workload: 16 independent 1024^2 meshes
+    (second-order momentum compaction, parallel-plates CSR plus resistive
+    wall, five-point Fokker-Planck, 4000 steps per period; currents
+    0.8-1.5x threshold, gaps 24-56 mm) in one batch after the autotuner:
+    whole-batch cell updates per second."""
+    import numpy as np
+    from inovesa_amd import Batch
+    from inovesa_amd.workloads import (impedances, initial_mesh,
+                                       production_config)
+    descs, inits, zws, zcs = [], [], [], []
+    for k, (f, g) in enumerate(zip(np.geomspace(0.8, 1.5, 16),
+                                   np.linspace(0.024, 0.056, 16))):
+        desc, d, rw = production_config(gap=float(g), current_factor=float(f))
+        zw, zc = impedances(d, resistive_wall=rw)
+        descs.append(desc)
+        inits.append(initial_mesh(desc["n"], seed=100 + k))
+        zws.append(zw)
+        zcs.append(zc)
+    b = Batch(descs, inits, zws, zcs)
+    table = b.autotune(steps=6)
+    cells = b.info()["cells"]
+    b.record(steps + 64)
+    b.run(64)
+    b.dev.sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.dev.sync()
+    dt = time.perf_counter() - t0
+    chosen = [(r["T"], r["threads"],
+               "persistent" if r["persistent"] else "per-tile",
+               round(r["us_per_step"], 2)) for r in table if r["chosen"]]
+    log(f"production: 16 x 1024^2, {steps} steps in {dt:.2f} s")
+    return {"value": cells * steps / dt, "unit": "cell-updates/s",
+            "instances": 16, "n": 1024, "cells_per_step": int(cells),
+            "steps": steps, "ms_per_step": dt * 1e3 / steps,
+            "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
+            "tiles_chosen": chosen[0] if chosen else None}
 
 
 def scan_ablation(targets, log):
@@ -247,6 +294,7 @@ def main():
            "other_projection": other_line}
     if not args.no_batch:
         out["batched"] = batched_throughput(args.batch_steps, log)
+        out["production"] = production_throughput(1000, log)
     if not args.no_scan:
         out["scan"] = scan_ablation(args.scan_targets, log)
     if not args.no_cpu:

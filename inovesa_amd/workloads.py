@@ -73,6 +73,34 @@ def kick_drift_config(n=512, steps_per_ts=2000, gap=0.032, current_factor=1.5,
     return desc, d
 
 
+def production_config(n=1024, steps_per_ts=4000, gap=0.032, alpha1=4e-4,
+                      current_factor=1.2, Ib=None, padding=8.0,
+                      resistive_wall=(1.4e6, 0.0)):
+    """Large production workload: the kick-drift maps with second-order
+    momentum compaction (alpha2 = 5 alpha1, three drift terms of
+    src/main.cpp:663-664), a five-point Fokker-Planck stencil, 4000 steps
+    per synchrotron period, and a wake impedance of parallel-plates CSR
+    plus the resistive wall (conductivity, permeability parameter) of
+    src/Z/ResistiveWall.cpp.  Returns (desc, derived, resistive_wall) for
+    impedances(derived, resistive_wall=...)."""
+    ring = dict(grid_size=n, steps_per_ts=steps_per_ts, vacuum_gap=gap,
+                padding=padding, alpha1=alpha1, alpha2=5.0 * alpha1)
+    base = derive(**ring)
+    d = derive(**ring, bunch_current=Ib if Ib is not None
+               else current_factor * base["Ith"])
+    desc = dict(n=n, qmin=d["qmin"], qmax=d["qmax"], pmin=d["pmin"],
+                pmax=d["pmax"], qscale=d["qscale"], pscale=d["pscale"],
+                scheme=0, interp=4, rot_clamp=0, angle=d["angle"],
+                rf_linear=0, rf_angle=d["angle"],
+                revolutionpart=d["revolutionpart"], V_RF=d["V_eff"],
+                f_RF=d["f_RF"], V0=d["V0"], slip=d["slip"],
+                E0=d["ring"]["beam_energy"], use_fp=1, fptype=3, e1=d["e1"],
+                deriv=5, use_wake=1, spacing_bins=d["spacing_bins"],
+                wakescaling=d["wakescaling"], use_csr=1,
+                fc=d["ring"]["cutoff_frequency"])
+    return desc, d, resistive_wall
+
+
 def fp_diffusion_number(n, e1, pq_size=12.0):
     """e1 / dp^2 of the FokkerPlanckMap diffusion stencil
     (src/SM/FokkerPlanckMap.cpp:38-118) on an n-point momentum axis of

@@ -93,6 +93,31 @@ def test_persistent_launch_equals_single_steps(V, tile, threads):
         assert np.array_equal(cb, cs), k
 
 
+def test_production_batch_equals_single_steps(V):
+    """Four instances of the 1024^2 production workload (currents and gaps
+    apart) in one batch, autotuned: each equals its run alone, bit for
+    bit."""
+    from inovesa_amd.workloads import impedances, initial_mesh, production_config
+    inst = []
+    for k, (f, g) in enumerate([(0.8, 0.024), (1.0, 0.032), (1.2, 0.04),
+                                (1.5, 0.056)]):
+        desc, d, rw = production_config(gap=g, current_factor=f)
+        zw, zc = impedances(d, resistive_wall=rw)
+        inst.append((desc, zw, zc, initial_mesh(desc["n"], seed=40 + k)))
+    b = _batch(V, inst)
+    b.autotune(steps=4)
+    steps = 66
+    b.record(steps + 4)
+    b.run(steps)
+    for k in (0, 3):
+        s = _single(V, inst[k])
+        s.record(steps + 4)
+        s.run(steps)
+        for x, y in zip(b.mesh(k), s.mesh()):
+            assert np.array_equal(x, y), k
+        assert np.array_equal(b.history(k)[0], s.history()[0]), k
+
+
 def test_retile_rejects_and_keeps_tiling(V):
     inst = _instances(V, (128,), 2)
     b = _batch(V, inst)

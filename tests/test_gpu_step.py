@@ -243,6 +243,42 @@ def test_history_and_drift_vs_fp64(V, orc, orc64, scheme, n, kw, fast):
     assert e64 <= 10 * o64 + 1e-5
 
 
+def test_production_workload_vs_oracle(V, orc, orc64):
+    """The large production workload (1024^2, second-order momentum
+    compaction, parallel-plates CSR plus resistive wall, five-point
+    Fokker-Planck stencil, 4000 steps per period): one step from the same
+    state within 1e-5 on mesh, profile and wake, and 100 steps against the
+    FP32 and FP64 oracles."""
+    from inovesa_amd.workloads import impedances, initial_mesh, production_config
+    desc, d, rw = production_config()
+    zw, zc = impedances(d, resistive_wall=rw)
+    init = initial_mesh(desc["n"], seed=5)
+    assert len(desc["slip"]) == 3 and desc["deriv"] == 5
+    st = V.Step(desc, init, zw, zc)
+    st.record(110, profiles=True)
+    st.run(1)
+    mesh, proj, wake = st.mesh()
+    sim = orc.sim(oracle_cfg(desc, zw.n), init, zw.data(), zc.data())
+    sim.run(1)
+    omesh, oproj, owake = sim.get()
+    assert rel_linf(wake, owake) <= 1e-5
+    assert rel_linf(mesh, omesh) <= 1e-5
+    assert rel_linf(proj, oproj) <= 1e-5
+    st.run(99)
+    sim.run(99)
+    sim64 = orc64.sim(oracle_cfg(desc, zw.n), init.astype(np.float64),
+                      zw.data(), zc.data())
+    sim64.run(100)
+    mesh, _, _ = st.mesh()
+    omesh, _, _ = sim.get()
+    m64, _, _ = sim64.get()
+    e32, e64, o64 = rel_linf(mesh, omesh), rel_linf(mesh, m64), rel_linf(omesh, m64)
+    print(f"\n1024^2 production workload, 100 steps: GPU vs FP32 oracle "
+          f"{e32:.2e}, GPU vs FP64 {e64:.2e}, FP32 oracle vs FP64 {o64:.2e}")
+    assert e32 <= 1e-4
+    assert e64 <= 10 * o64 + 1e-5
+
+
 def test_step_rejects_bad_descriptions(V):
     desc, d, zw, zc, init = _setup(V, 0, 128, steps_per_ts=300)
     with pytest.raises(V.VfpsInvalid):
