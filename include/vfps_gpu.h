@@ -434,11 +434,13 @@ VFPS_API int vfps_batch_timing(vfps_batch* b, uint32_t nsteps, float* ms_step);
 /* Autotuner: per mesh size, time every tile height T in {1,2,4,8,16},
  * workgroup size in {128,256,512} and launch form (one workgroup per tile,
  * or persistent: as many workgroups as stay resident, each walking several
- * tiles) that fits (steps steps each, on that size's instances alone) and
- * keep the fastest.  The instances' state is restored afterwards; results
- * do not depend on the choice.  table (cap rows of VFPS_TUNE_COLS floats:
- * n, T, threads, us/step, chosen, persistent) may be NULL. */
-#define VFPS_TUNE_COLS 6
+ * tiles) that fits (steps steps each, on that size's instances alone),
+ * time the three fastest again over 4 x steps and keep the fastest of
+ * those.  The instances' state is restored afterwards; results do not
+ * depend on the choice.  table (cap rows of VFPS_TUNE_COLS floats: n, T,
+ * threads, us/step (of the longer timing where re-timed), chosen,
+ * persistent, re-timed) may be NULL. */
+#define VFPS_TUNE_COLS 7
 VFPS_API int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                                  uint32_t cap, uint32_t* nrows);
 

@@ -724,15 +724,16 @@ class Batch:
     def autotune(self, steps=20):
         """Pick the tile height, workgroup size and launch form (persistent
         or one workgroup per tile) per mesh size; returns the timing table
-        (n, T, threads, us/step, chosen, persistent)."""
+        (n, T, threads, us/step, chosen, persistent, retimed: the three
+        fastest of a size are timed again over 4 x steps)."""
         cap = 512
-        table = np.zeros((cap, 6), np.float32)
+        table = np.zeros((cap, 7), np.float32)
         rows = C.c_uint32()
         call("vfps_batch_autotune", self.h, steps, _p(table), cap,
              C.byref(rows))
         return [dict(n=int(r[0]), T=int(r[1]), threads=int(r[2]),
                      us_per_step=float(r[3]), chosen=bool(r[4]),
-                     persistent=bool(r[5]))
+                     persistent=bool(r[5]), retimed=bool(r[6]))
                 for r in table[:min(rows.value, cap)]]
 
     def __del__(self):
