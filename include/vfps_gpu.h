@@ -409,6 +409,11 @@ VFPS_API int vfps_batch_get_csr_histories(vfps_batch* b, uint32_t first,
 /* instances, launch groups and mesh cells per step */
 VFPS_API int vfps_batch_info(vfps_batch* b, uint32_t* ninst,
                              uint32_t* ngroups, uint64_t* cells);
+/* merge != 0: one launch per kernel variant, workgroup size and launch
+ * form, whatever the tiles' LDS (each workgroup gets the largest); 0 (the
+ * default): also one per LDS size class, the launches as parallel
+ * branches.  Results do not depend on it. */
+VFPS_API int vfps_batch_set_merge(vfps_batch* b, int merge);
 /* vfps_batch_retile for every instance of mesh size n at once (one
  * rebuild of the launches); on failure the previous tiling stays */
 VFPS_API int vfps_batch_retile_size(vfps_batch* b, uint32_t n,

@@ -53,6 +53,9 @@ struct vfps_batch {
     vfps_dev* dev = nullptr;
     std::vector<vfps_step*> members;
     std::vector<Group> groups;
+    bool merge = false;   // one launch per variant, workgroup size and form
+                          // (LDS classes merged: every tile gets the largest
+                          // tile's LDS)
     hipEvent_t fork = nullptr;
     uint32_t parity = 0;
     hipGraphExec_t graph = nullptr;
@@ -162,7 +165,7 @@ int build_groups(vfps_batch* b)
         Group* g = nullptr;
         for (auto& e : b->groups) {
             if (same_variant(e.variant, s->variant) && e.threads == s->threads &&
-                e.lds_class == cls && e.persist == s->persist) {
+                (b->merge || e.lds_class == cls) && e.persist == s->persist) {
                 g = &e;
                 break;
             }
@@ -474,6 +477,17 @@ int vfps_batch_retile(vfps_batch* b, uint32_t inst, uint32_t tile_rows,
         return rc;
     }
     return VFPS_OK;
+    VFPS_TRY_END
+}
+
+int vfps_batch_set_merge(vfps_batch* b, int merge)
+{
+    VFPS_CHECK_ARG(b, "vfps_batch_set_merge: NULL batch");
+    VFPS_TRY_BEGIN
+    VFPS_HIP(hipStreamSynchronize(b->dev->stream));
+    sync_parity(b);
+    b->merge = merge != 0;
+    return build_groups(b);
     VFPS_TRY_END
 }
 

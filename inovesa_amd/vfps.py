@@ -706,6 +706,11 @@ class Batch:
         return [dict(threads=int(r[0]), tiles=int(r[1]), workgroups=int(r[2]),
                      lds_bytes=int(r[3])) for r in out[:min(ng.value, cap)]]
 
+    def set_merge(self, merge):
+        """One launch per step for all size classes (merge=True) or one per
+        LDS class as parallel branches (False, the default)."""
+        call("vfps_batch_set_merge", self.h, 1 if merge else 0)
+
     def retile_size(self, n, tile_rows, threads, persistent=False):
         """Set the tiling of every instance of mesh size n."""
         call("vfps_batch_retile_size", self.h, n, tile_rows, threads,

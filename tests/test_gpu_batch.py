@@ -118,6 +118,21 @@ def test_production_batch_equals_single_steps(V):
         assert np.array_equal(b.history(k)[0], s.history()[0]), k
 
 
+def test_merged_launch_equals_split(V):
+    """One launch for all size classes (every tile with the largest LDS)
+    against one per LDS class: the same results bit for bit."""
+    inst = _instances(V, (128, 256, 384), 2)
+    a = _batch(V, inst)
+    b = _batch(V, inst)
+    b.set_merge(True)
+    assert len(b.launches()) < len(a.launches())
+    a.run(66)
+    b.run(66)
+    for k in range(len(inst)):
+        for x, y in zip(a.mesh(k), b.mesh(k)):
+            assert np.array_equal(x, y), k
+
+
 def test_retile_rejects_and_keeps_tiling(V):
     inst = _instances(V, (128,), 2)
     b = _batch(V, inst)
