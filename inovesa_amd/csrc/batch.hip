@@ -644,6 +644,13 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
         uint32_t row;
         bool refined;
     };
+    // per class: the chosen tiling and the fastest per-tile one
+    struct Choice {
+        uint32_t n;
+        Cand best, best_tile;
+        bool has_tile;
+    };
+    std::vector<Choice> choices;
     for (auto& cl : classes) {
         const uint32_t n = cl.first;
         std::vector<Cand> cands;
@@ -682,6 +689,16 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
         for (const auto& c : cands) {
             if (c.refined && (!best || c.us < best->us)) best = &c;
         }
+        if (best) {
+            Choice ch{n, *best, *best, false};
+            for (const auto& c : cands) {
+                if (!c.pers && (!ch.has_tile || c.us < ch.best_tile.us)) {
+                    ch.best_tile = c;
+                    ch.has_tile = true;
+                }
+            }
+            choices.push_back(ch);
+        }
         for (uint32_t i : cl.second) {
             vfps_step* m = b->members[i];
             if (best) {
@@ -706,6 +723,64 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
             }
         }
         if (rc != VFPS_OK) break;
+    }
+    // A persistent launch holds every resident slot, so it can starve the
+    // other size classes' launches, which run beside it: with several
+    // classes, a persistent choice must also win on the whole batch.
+    if (rc == VFPS_OK && choices.size() > 1) {
+        auto time_all = [&](float* us) -> int {
+            vfps_batch sub;
+            sub.dev = b->dev;
+            sub.fork = b->fork;
+            sub.members = b->members;
+            int r1 = build_groups(&sub);
+            float ms = 0;
+            if (r1 == VFPS_OK) {
+                for (uint32_t k = 0; k < 2 && r1 == VFPS_OK; k++) {
+                    r1 = launch_step(&sub, (par + k) & 1u);
+                }
+                hipEventRecord(e0, st);
+                for (uint32_t k = 0; k < 4 * even && r1 == VFPS_OK; k++) {
+                    r1 = launch_step(&sub, (par + k) & 1u);
+                }
+                hipEventRecord(e1, st);
+                hipEventSynchronize(e1);
+                hipEventElapsedTime(&ms, e0, e1);
+            }
+            free_groups(&sub);
+            sub.fork = nullptr;
+            sub.members.clear();
+            *us = 1e3f * ms / (float)(4 * even);
+            return r1;
+        };
+        for (auto& ch : choices) {
+            if (!ch.best.pers || !ch.has_tile || rc != VFPS_OK) continue;
+            float t_pers = 0, t_tile = 0;
+            if ((rc = time_all(&t_pers)) != VFPS_OK) break;
+            for (uint32_t i : classes[ch.n]) {
+                b->members[i]->persist = false;
+                if ((rc = step_retile(b->members[i], ch.best_tile.T,
+                                      ch.best_tile.thr)) != VFPS_OK) break;
+            }
+            if (rc != VFPS_OK) break;
+            if ((rc = time_all(&t_tile)) != VFPS_OK) break;
+            if (t_pers < t_tile) {
+                // back to the persistent choice
+                for (uint32_t i : classes[ch.n]) {
+                    b->members[i]->persist = true;
+                    if ((rc = step_retile(b->members[i], ch.best.T,
+                                          ch.best.thr)) != VFPS_OK) break;
+                }
+            } else if (table) {
+                // the chosen mark moves to the per-tile tiling
+                if (ch.best.row < cap) {
+                    table[VFPS_TUNE_COLS * (size_t)ch.best.row + 4] = 0.0f;
+                }
+                if (ch.best_tile.row < cap) {
+                    table[VFPS_TUNE_COLS * (size_t)ch.best_tile.row + 4] = 1.0f;
+                }
+            }
+        }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);

@@ -26,6 +26,37 @@ __device__ __forceinline__ T wave_max(T v)
     return v;
 }
 
+// One DPP move of a double: dpp_ctrl applied to both 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF,
+                                               0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF,
+                                               0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum of a double over the 64 lanes, every lane receives it.  Within each
+// row of 16 lanes: neighbours (quad_perm 1,0,3,2), pairs (quad_perm
+// 2,3,0,1), half-row mirror, row mirror; then the rows' sums read from
+// lanes 0, 16, 32, 48 and added in that order.  Register moves only (no
+// LDS permutes); the order is fixed, so every workgroup shape gives the
+// same bits.
+__device__ __forceinline__ double wave_sum_dpp(double v)
+{
+    v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_move<0x141>(v);  // row_half_mirror
+    v += dpp_move<0x140>(v);  // row_mirror
+    auto lane = [&](int l) {
+        const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+        return __hiloint2double(hi, lo);
+    };
+    return ((lane(0) + lane(16)) + lane(32)) + lane(48);
+}
+
 // Sum over the workgroup; `part` holds one entry per wavefront (<= 16).
 // Every thread receives the total.  Ends with a barrier, so the scratch
 // can be reused right away.

@@ -63,7 +63,7 @@ __host__ __device__ inline uint32_t step_round4(uint32_t v) { return (v + 3u) & 
 // The direct form's dot products (one per wake row, CSR row and, for the
 // rotation, row 0) are each summed by one wavefront in a fixed order that
 // depends on neither the tile height nor the workgroup size: lane l takes
-// x' = l, l + 64, ... in increasing order, then a butterfly over the lanes.
+// x' = l, l + 64, ... in increasing order, then wave_sum_dpp over the lanes.
 // LDS words of one row's kick taps
 constexpr uint32_t STEP_TAP_WORDS = 8;
 
@@ -557,7 +557,7 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             for (uint32_t x = lane; x < n; x += 64) {
                 acc += win[base + n - 1 - x] * (double)pprev[x];
             }
-            acc = wave_sum(acc);
+            acc = wave_sum_dpp(acc);
             if (lane != 0) {
                 continue;
             }

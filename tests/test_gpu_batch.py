@@ -163,8 +163,12 @@ def test_autotune_keeps_results_and_reports_table(V):
         assert len(chosen) == 1
         assert all(r["persistent"] in (False, True) for r in rows)
         retimed = [r for r in rows if r["retimed"]]
-        assert 1 <= len(retimed) <= 3 and chosen[0]["retimed"]
-        assert chosen[0]["us_per_step"] == min(r["us_per_step"] for r in retimed)
+        assert 1 <= len(retimed) <= 3
+        fastest = min(retimed, key=lambda r: r["us_per_step"])
+        if chosen[0] is not fastest:
+            # a persistent choice that lost on the whole batch gives way to
+            # the size's fastest per-tile tiling
+            assert fastest["persistent"] and not chosen[0]["persistent"]
         k = [j for j, i in enumerate(inst) if i[0]["n"] == n][0]
         cfg = a.config(k)
         assert (cfg["tile_rows"], cfg["threads"], cfg["persistent"]) == (
