@@ -223,6 +223,74 @@ __device__ __forceinline__ void kick_rows(const float* __restrict__ src,
     }
 }
 
+// The wake kick and then the RF kick along y on rows [r0, r1), fused: each
+// thread makes four consecutive outputs of the second kick from the IT+3
+// wake-kicked samples they read, and those from 2 IT + 2 consecutive
+// inputs.  Every wake-kicked sample is the same sum, in the same order, as
+// kick_rows makes it (a sample out of range reads as zero in both), so the
+// result equals the two kicks one after the other bit for bit, without the
+// intermediate rows in LDS or the barrier between the two.
+template <int IT>
+__device__ __forceinline__ void kick2_rows(const float* __restrict__ src,
+                                           float* __restrict__ dst, uint32_t n,
+                                           int r0, int r1,
+                                           const float* __restrict__ taps1,
+                                           const float* __restrict__ taps2)
+{
+    if (r1 <= r0) {
+        return;
+    }
+    const uint32_t bs = blockDim.x;
+    const uint32_t ng = n >> 2;
+    const uint32_t drow = bs / ng, dg = bs % ng;
+    uint32_t row = (uint32_t)r0 + threadIdx.x / ng;
+    uint32_t g = threadIdx.x % ng;
+    while (row < (uint32_t)r1) {
+        const float* t1 = taps1 + row * STEP_TAP_WORDS;
+        const float* t2 = taps2 + row * STEP_TAP_WORDS;
+        const vec4f w1 = *reinterpret_cast<const vec4f*>(t1);
+        const vec4f w2 = *reinterpret_cast<const vec4f*>(t2);
+        const int32_t base1 = __float_as_int(t1[4]);
+        const int32_t base2 = __float_as_int(t2[4]);
+        const float* s = src + (size_t)row * n;
+        const int32_t y0 = (int32_t)(4 * g);
+        float v[2 * IT + 2];
+#pragma unroll
+        for (int k = 0; k < 2 * IT + 2; k++) {
+            const uint32_t ys = (uint32_t)(y0 + base2 + base1 + k);
+            v[k] = ys < n ? s[ys] : 0.0f;
+        }
+        float c[IT + 3];
+#pragma unroll
+        for (int k = 0; k < IT + 3; k++) {
+            const uint32_t yc = (uint32_t)(y0 + base2 + k);
+            float acc = 0.0f;
+#pragma unroll
+            for (int j = 0; j < IT; j++) {
+                acc = acc + v[k + j] * w1[j];
+            }
+            c[k] = yc < n ? acc : 0.0f;
+        }
+        vec4f o;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int j = 0; j < IT; j++) {
+                acc = acc + c[e + j] * w2[j];
+            }
+            o[e] = acc;
+        }
+        *reinterpret_cast<vec4f*>(dst + (size_t)row * n + 4 * g) = o;
+        g += dg;
+        row += drow;
+        if (g >= ng) {
+            g -= ng;
+            row++;
+        }
+    }
+}
+
 // Workgroup-shared scalars of one tile
 struct StepShared {
     double red[16];
@@ -604,8 +672,15 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
 
     float* cur = bufA;
     float* oth = bufB;
-    // ---- C: wake kick along y ----------------------------------------------
-    if (wake) {
+    // ---- C, D: wake kick and RF kick along y -------------------------------
+    if (SCHEME == 0 && wake) {
+        // both kicks in one pass (scheme 0)
+        kick2_rows<IT>(cur, oth, n, lr0, lr1, tapw, tapr);
+        __syncthreads();
+        float* t = cur;
+        cur = oth;
+        oth = t;
+    } else if (wake) {
         kick_rows<IT>(cur, oth, n, lr0, lr1, tapw);
         __syncthreads();
         float* t = cur;
@@ -614,12 +689,14 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
     }
     VFPS_STAMP(3);
     if constexpr (SCHEME == 0) {
-        // ---- D: RF kick along y --------------------------------------------
-        kick_rows<IT>(cur, oth, n, lr0, lr1, tapr);
-        __syncthreads();
-        float* t = cur;
-        cur = oth;
-        oth = t;
+        if (!wake) {
+            // ---- D: RF kick along y ----------------------------------------
+            kick_rows<IT>(cur, oth, n, lr0, lr1, tapr);
+            __syncthreads();
+            float* t = cur;
+            cur = oth;
+            oth = t;
+        }
         VFPS_STAMP(4);
         // ---- E: drift along x, one column per thread ----------------------
 #pragma unroll
