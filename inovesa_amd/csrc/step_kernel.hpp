@@ -707,6 +707,29 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             }
             const uint32_t* idx = col_didx[c];
             const float* w = col_dw[c];
+            // fast path: every tap of every output row lies in the mesh and
+            // in the buffer rows, so no tap needs a check
+            int32_t tmin = (int32_t)idx[0], tmax = (int32_t)idx[0];
+#pragma unroll
+            for (int j = 1; j < IT; j++) {
+                tmin = min(tmin, (int32_t)idx[j]);
+                tmax = max(tmax, (int32_t)idx[j]);
+            }
+            const int32_t xlo = x0 + tmin - half;
+            const int32_t xhi = x0 + rows_out - 1 + tmax - half;
+            if (xlo >= 0 && xhi < (int32_t)n && xlo - rbase >= lr0 &&
+                xhi - rbase < lr1) {
+                for (int orow = 0; orow < rows_out; orow++) {
+                    const int32_t xb = x0 + orow - half - rbase;
+                    float v = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < IT; j++) {
+                        v = v + cur[(size_t)(xb + (int32_t)idx[j]) * n + y] * w[j];
+                    }
+                    oth[(size_t)orow * n + y] = v;
+                }
+                continue;
+            }
             for (int orow = 0; orow < rows_out; orow++) {
                 const uint32_t x = (uint32_t)(x0 + orow);
                 float v = 0.0f;
