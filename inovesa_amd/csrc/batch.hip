@@ -110,8 +110,7 @@ int build_group(vfps_batch* b, Group& g)
         for (uint32_t t = 0; t < s->tiles(); t++) {
             map.push_back((k << 16) | t);
         }
-        g.lds = std::max(g.lds, step_lds_bytes(s->variant, s->desc.n, s->T,
-                                               s->stride));
+        g.lds = std::max(g.lds, step_lds_of(s));
     }
     while (map.size() % 8u != 0u) {
         map.push_back(STEP_NO_TILE);
@@ -159,9 +158,7 @@ int build_groups(vfps_batch* b)
     free_groups(b);
     for (uint32_t i = 0; i < b->members.size(); i++) {
         const vfps_step* s = b->members[i];
-        const uint32_t cls = ceil_div(step_lds_bytes(s->variant, s->desc.n,
-                                                     s->T, s->stride),
-                                      8192);
+        const uint32_t cls = ceil_div(step_lds_of(s), 8192);
         Group* g = nullptr;
         for (auto& e : b->groups) {
             if (same_variant(e.variant, s->variant) && e.threads == s->threads &&
@@ -286,6 +283,7 @@ int vfps_batch_create(vfps_dev* dev, uint32_t ninst,
         int rc = vfps_step_create(dev, &d, inits[i], zwake ? zwake[i] : nullptr,
                                   zcsr ? zcsr[i] : nullptr, &s);
         if (rc != VFPS_OK) return rc;
+        s->in_batch = true;
         b->members.push_back(s);
         VFPS_CHECK_ARG(s->tiles() <= 0xFFFFu, "vfps_batch_create: too many tiles");
     }

@@ -65,10 +65,16 @@ const void* step_kernel_for(const StepVariant& v, int form, uint32_t maxc)
 }
 
 size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
-                      uint32_t stride)
+                      uint32_t stride, bool one_mesh)
 {
     return sizeof(float) *
-           step_lds(n, T, stride, v.direct, (uint32_t)v.it, v.scheme).words;
+           step_lds(n, T, stride, v.direct, (uint32_t)v.it, v.scheme, one_mesh)
+               .words;
+}
+
+size_t step_lds_of(const vfps_step* s)
+{
+    return step_lds_bytes(s->variant, s->desc.n, s->T, s->stride, !s->in_batch);
 }
 
 int step_set_lds_limit(const StepVariant& v, size_t bytes)
@@ -332,7 +338,7 @@ static int step_launch_fused(vfps_step* s, uint32_t parity, hipStream_t st)
     }
     return launch_step_kernel(s->variant, s->inst_d, nullptr, s->tiles(),
                               step_lds_bytes(s->variant, s->desc.n, s->T,
-                                             s->stride),
+                                                   s->stride, true),
                               s->threads, s->desc.n, parity, st, &s->inst_h);
 }
 
@@ -619,7 +625,7 @@ int step_retile(vfps_step* s, uint32_t T, uint32_t threads)
     s->T = T;
     s->threads = threads;
     int rc = step_upload_desc(s);
-    const size_t lds = step_lds_bytes(s->variant, s->desc.n, s->T, s->stride);
+    const size_t lds = step_lds_of(s);
     if (rc == VFPS_OK && lds > s->dev->lds_max) {
         rc = fail(VFPS_ERR_INVALID, "retile: tile exceeds the LDS of one CU");
     }
@@ -823,11 +829,11 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
     if ((rc = step_upload_desc(s)) != VFPS_OK) return rc;
     if (s->fused) {
         while (s->T > 1 &&
-               step_lds_bytes(s->variant, n, s->T, s->stride) > dev->lds_max) {
+               step_lds_bytes(s->variant, n, s->T, s->stride, true) > dev->lds_max) {
             s->T >>= 1;
             if ((rc = step_upload_desc(s)) != VFPS_OK) return rc;
         }
-        const size_t lds = step_lds_bytes(s->variant, n, s->T, s->stride);
+        const size_t lds = step_lds_bytes(s->variant, n, s->T, s->stride, true);
         VFPS_CHECK_ARG(lds <= dev->lds_max,
                        "vfps_step_create: mesh row too long for the LDS of one "
                        "CU (use fused = 0)");
@@ -1008,9 +1014,7 @@ int vfps_step_config(vfps_step* s, uint32_t* out, uint32_t nout)
     out[2] = s->T;
     out[3] = s->threads;
     out[4] = s->stride;
-    out[5] = s->fused ? (uint32_t)step_lds_bytes(s->variant, s->desc.n, s->T,
-                                                 s->stride)
-                      : 0u;
+    out[5] = s->fused ? (uint32_t)step_lds_of(s) : 0u;
     if (nout >= 7) {
         out[6] = s->persist ? 1u : 0u;
     }
@@ -1125,7 +1129,7 @@ int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
         for (uint32_t k = 0; k < nsteps && rc == VFPS_OK; k++) {
             rc = launch_step_kernel(s->variant, s->inst_d, nullptr, s->tiles(),
                                     step_lds_bytes(s->variant, s->desc.n, s->T,
-                                                   s->stride),
+                                                   s->stride, true),
                                     s->threads, s->desc.n, s->parity, st,
                                     &s->inst_h);
             s->parity ^= 1u;
@@ -1146,8 +1150,8 @@ int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
             if (rc == VFPS_OK) {
                 rc = launch_step_kernel(s->variant, s->inst_d, nullptr,
                                         s->tiles(),
-                                        step_lds_bytes(s->variant, s->desc.n,
-                                                       s->T, s->stride),
+                                        step_lds_bytes(s->variant, s->desc.n, s->T,
+                                                   s->stride, true),
                                         s->threads, s->desc.n, par, st,
                                         &s->inst_h);
             }

@@ -57,8 +57,10 @@ struct StepVariant {
 
 // form: StepForm (step_kernel.hpp); maxc: columns per thread
 const void* step_kernel_for(const StepVariant& v, int form, uint32_t maxc);
+// one_mesh: the one-mesh form's layout (kernel windows in LDS), else a
+// batch's
 size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
-                      uint32_t stride);
+                      uint32_t stride, bool one_mesh);
 int step_set_lds_limit(const StepVariant& v, size_t bytes);
 // Launch the step kernel: `nblocks` workgroups of `threads` with
 // `lds_bytes` of dynamic LDS (at least step_lds_bytes of every instance);
@@ -135,6 +137,7 @@ struct vfps_step {
     vfps_impl::StepVariant variant{};
     uint32_t T = 1, threads = 256, stride = 1, hmax = 0, M = 0;
     bool persist = false;  // in a batch: persistent launch (grid < tiles)
+    bool in_batch = false; // a batch member (the batch forms' LDS layout)
     bool fused = true;
     bool direct = true;
     uint32_t parity = 0;  // fused path: buffer holding the state (0: t1)
@@ -147,6 +150,8 @@ struct vfps_step {
 
 namespace vfps_impl {
 int step_retile(vfps_step* s, uint32_t T, uint32_t threads);
+// LDS bytes of the step's tile in the form it runs in
+size_t step_lds_of(const vfps_step* s);
 }  // namespace vfps_impl
 
 namespace vfps_impl {

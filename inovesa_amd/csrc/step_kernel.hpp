@@ -71,21 +71,26 @@ constexpr uint32_t STEP_TAP_WORDS = 8;
 constexpr uint32_t STEP_TILE_IN_BUF = 0xFFFFFFFFu;
 
 // scheme 1 (rotation) with the direct wake also keeps input row 0 and the
-// kernel window of its wake offset
+// kernel window of its wake offset.  windows: the FP64 kernel windows live
+// in LDS (the one-mesh form, where they load with the mesh); batches read
+// them from global memory (tables in L2 shared by every tile) and spend
+// the LDS on occupancy instead.
 __host__ __device__ inline StepLds step_lds(uint32_t n, uint32_t T, uint32_t stride,
                                             bool direct, uint32_t it,
-                                            int scheme)
+                                            int scheme, bool windows)
 {
     StepLds L{};
     uint32_t w = 0;
     if (direct) {
-        L.gwin = w;
-        w += 2 * (n + stride);
-        L.hwin = w;
-        w += 2 * (n + T);
+        if (windows) {
+            L.gwin = w;
+            w += 2 * (n + stride);
+            L.hwin = w;
+            w += 2 * (n + T);
+        }
         L.csr_rows = w;
         w += 2 * T;
-        if (scheme == 1) {
+        if (scheme == 1 && windows) {
             L.g0win = w;
             w += 2 * n;
         }
@@ -335,7 +340,7 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
     // rows of the buffer that exist in the mesh
     const int lr0 = max(0, -rbase);
     const int lr1 = min(R, (int)n - rbase);
-    const StepLds L = step_lds(n, T, stride, DIRECT, IT, SCHEME);
+    const StepLds L = step_lds(n, T, stride, DIRECT, IT, SCHEME, ONE_MESH);
     float* bufA = lds + L.bufA;
     float* bufB = lds + L.bufB;
     // placed after the drift/rotation, which frees the buffer it read
@@ -423,14 +428,14 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             const uint32_t x = tid + (uint32_t)c * bs;
             pv[c] = proj_in[x < n ? x : n - 1u];
         }
-        if (wake) {
+        if (ONE_MESH && wake) {
 #pragma unroll
             for (int c = 0; c < WV; c++) {
                 const uint32_t k = tid + (uint32_t)c * bs;
                 gv[c] = wake_g[g0 + (k < gl ? k : gl - 1u)];
             }
         }
-        if (csr) {
+        if (ONE_MESH && csr) {
 #pragma unroll
             for (int c = 0; c < WV; c++) {
                 const uint32_t k = tid + (uint32_t)c * bs;
@@ -443,7 +448,7 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                 const uint32_t y = tid + (uint32_t)c * bs;
                 const uint32_t yy = y < n ? y : n - 1u;
                 r0v[c] = in[yy];
-                g0v[c] = wake_g[yy];
+                if (ONE_MESH) g0v[c] = wake_g[yy];
             }
         }
     }
@@ -507,14 +512,14 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             const uint32_t x = tid + (uint32_t)c * bs;
             if (x < n) pprev[x] = pv[c];
         }
-        if (wake) {
+        if (ONE_MESH && wake) {
 #pragma unroll
             for (int c = 0; c < WV; c++) {
                 const uint32_t k = tid + (uint32_t)c * bs;
                 if (k < gl) gwin[k] = gv[c];
             }
         }
-        if (csr) {
+        if (ONE_MESH && csr) {
 #pragma unroll
             for (int c = 0; c < WV; c++) {
                 const uint32_t k = tid + (uint32_t)c * bs;
@@ -529,7 +534,7 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                 const uint32_t y = tid + (uint32_t)c * bs;
                 if (y < n) {
                     row0[y] = r0v[c];
-                    g0win[y] = g0v[c];
+                    if (ONE_MESH) g0win[y] = g0v[c];
                 }
             }
         }
@@ -610,20 +615,36 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
         const int ncsr = csr ? rows_out : 0;
         const int J = nwk + ncsr + ((need_s00 && wake) ? 1 : 0);
         for (int job = (int)wave; job < J; job += (int)nw) {
-            const double* win = gwin;
             uint32_t base = 0;
+            int which = 0;  // 0 wake row, 1 CSR row, 2 row 0's wake
             if (job < nwk) {
                 base = (uint32_t)job;
             } else if (job < nwk + ncsr) {
-                win = hwin;
+                which = 1;
                 base = (uint32_t)(job - nwk);
             } else {
-                win = reinterpret_cast<const double*>(lds + L.g0win);
+                which = 2;
             }
             double acc = 0.0;
+            if constexpr (ONE_MESH) {
+                const double* win =
+                    which == 0 ? gwin
+                               : (which == 1 ? hwin
+                                             : reinterpret_cast<const double*>(
+                                                   lds + L.g0win));
 #pragma unroll 4
-            for (uint32_t x = lane; x < n; x += 64) {
-                acc += win[base + n - 1 - x] * (double)pprev[x];
+                for (uint32_t x = lane; x < n; x += 64) {
+                    acc += win[base + n - 1 - x] * (double)pprev[x];
+                }
+            } else {
+                // the same windows, read from the tables in global memory
+                const gdouble* win =
+                    which == 0 ? wake_g + g0
+                               : (which == 1 ? csr_h + (uint32_t)x0 : wake_g);
+#pragma unroll 4
+                for (uint32_t x = lane; x < n; x += 64) {
+                    acc += win[base + n - 1 - x] * (double)pprev[x];
+                }
             }
             acc = wave_sum_dpp(acc);
             if (lane != 0) {
