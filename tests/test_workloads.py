@@ -29,3 +29,31 @@ def test_reference_fp_update_bound(orc, damping, bounded):
     mesh, _, _ = sim.get()
     peak = float(np.abs(mesh).max())
     assert (peak < 0.2) == bounded, peak
+
+
+def test_production_config_terms():
+    """The large production workload: three drift terms with alpha2/alpha1
+    = 5 (slip_i = alpha_i / alpha0 x angle, src/main.cpp:663-664), the
+    five-point Fokker-Planck stencil, 1024^2 and 4000 steps per period."""
+    from inovesa_amd.workloads import production_config
+    desc, d, rw = production_config(alpha1=4e-4)
+    slip = desc["slip"]
+    assert desc["n"] == 1024 and desc["deriv"] == 5 and desc["scheme"] == 0
+    assert len(slip) == 3
+    alpha0 = d["ring"]["alpha0"]
+    np.testing.assert_allclose(slip[1] / slip[0], 4e-4 / alpha0, rtol=1e-6)
+    np.testing.assert_allclose(slip[2] / slip[1], 5.0, rtol=1e-6)
+    assert d["ring"]["steps_per_ts"] == 4000 and rw[0] > 0
+
+
+def test_batch_class():
+    from inovesa_amd.scan import batch_class
+    assert [batch_class(c) for c in (1, 16, 17, 207, 768, 1024, 1025)] == \
+        [16, 16, 32, 256, 1024, 1024, 2048]
+
+
+def test_scan_setup_detector_threshold_per_scheme():
+    from inovesa_amd.scan import ScanSetup
+    assert ScanSetup().tol == 1e-3
+    assert ScanSetup(scheme="rotation").tol == 5e-3
+    assert ScanSetup(tol=2e-3).tol == 2e-3
