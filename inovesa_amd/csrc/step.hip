@@ -129,6 +129,9 @@ int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
         return fail(VFPS_ERR_INVALID, "fused step: one mesh runs one "
                                       "workgroup per tile");
     }
+    if (single && v.direct) {
+        grid = nblocks + 1u;  // and one that closes the step
+    }
     if (threads == 0 || threads > STEP_MAX_THREADS || threads % 64u != 0u) {
         return fail(VFPS_ERR_INVALID, "fused step: workgroup size");
     }

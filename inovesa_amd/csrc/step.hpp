@@ -39,8 +39,9 @@ struct InstDesc {
     float* csr_hist;              // optional per-step CSR power history
     uint32_t hist_cap;
     uint32_t* counter;            // [2] completed steps, read at parity p,
-                                  // written at p^1 by tile 0 (direct form)
-                                  // or by the field kernel
+                                  // written at p^1 by the workgroup that
+                                  // closes the step (direct form) or by
+                                  // the field kernel
     uint32_t* err;                // set if a tap falls outside the halo
     uint32_t seq_proj;            // 1: x projection in the reference's order
     unsigned long long* tstamp;   // optional timestamps (phases of tile 0,
@@ -68,7 +69,8 @@ int step_set_lds_limit(const StepVariant& v, size_t bytes);
 // workgroup (STEP_NO_TILE: padding), else instance 0, tile = workgroup.
 // maxn: the largest mesh size of the launch (sets the columns per thread).
 // One mesh: desc_h (its descriptor, passed to the kernel by value), no
-// block map, one workgroup per tile.  A batch: blockmap_d of nblocks
+// block map, one workgroup per tile (direct wake: and one more that closes
+// the step).  A batch: blockmap_d of nblocks
 // entries (a multiple of 8); grid (a multiple of 8, at most nblocks; 0:
 // nblocks) workgroups walk it, several tiles each when grid < nblocks.
 int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
@@ -89,7 +91,7 @@ int launch_fields(bool wake, bool csr, const FieldParams* params_d,
                   uint32_t ninst, uint32_t Mmax, uint32_t parity,
                   hipStream_t st);
 // close the last step of a direct-form run: its CSR power from the tiles'
-// shares (the next step's tile 0 does this otherwise)
+// shares (the next step's closing workgroup does this otherwise)
 int launch_step_finalize(const InstDesc* descs_d, uint32_t ninst,
                          uint32_t parity, hipStream_t st);
 

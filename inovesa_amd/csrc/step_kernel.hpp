@@ -17,9 +17,10 @@
 //      products mesh*weight of the x projection;
 //   G  the x projection of its rows (PhaseSpace::updateXProjection), summed
 //      in std::inner_product order;
-//   H  (direct wake) every tile stores its CSR share; tile 0 counts the
-//      step, computes the FP64 charge integral and sums the previous step's
-//      shares in a fixed order (the launch boundary orders them).
+//   H  (direct wake) every tile stores its CSR share; tile 0 (a one-mesh
+//      launch: a workgroup of its own) counts the step, computes the FP64
+//      charge integral and sums the previous step's shares in a fixed order
+//      (the launch boundary orders them).
 // The mesh makes one trip HBM -> LDS -> HBM per step.  The state buffers
 // and the x projections ping-pong (`parity`), so a step reads only what
 // the previous launch wrote.
@@ -302,8 +303,52 @@ struct StepShared {
     float s00;
     uint32_t slot;
     uint32_t cnt;
-    uint32_t cvalid;  // step whose CSR shares the previous launch left
 };
+
+// Close a step of the direct form: count it, integrate the profile that
+// entered it (FP64, Simpson weights) and sum the previous launch's CSR
+// shares in a fixed order.  It reads only what earlier launches wrote, so
+// no workgroup of the launch waits for another.  cnt: the steps completed
+// before this one; red: 16 doubles of LDS.  Uniform over the workgroup.
+__device__ __forceinline__ void step_close(const InstDesc& d, uint32_t parity,
+                                           uint32_t cnt, double* red)
+{
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t n = d.n;
+    const bool csr = d.use_csr != 0;
+    const gfloat* pin = (const gfloat*)d.proj[parity];
+    const gfloat* ws = (const gfloat*)d.ws;
+    double f = 0.0, p = 0.0;
+    for (uint32_t x = tid; x < n; x += bs) {
+        f += (double)pin[x] * (double)ws[x];
+    }
+    const bool prev = csr && cnt >= 1 &&
+                      ((const guint*)d.csr_valid)[parity ^ 1u] == cnt;
+    if (prev) {
+        const gdouble* shares =
+            (const gdouble*)d.csr_part + (size_t)(parity ^ 1u) * n;
+        for (uint32_t k = tid; k < d.tiles; k += bs) {
+            p += shares[k];
+        }
+    }
+    block_sum2(f, p, red);
+    if (tid == 0) {
+        if (prev) {
+            *d.csr_power = (float)p;
+            if (d.csr_hist && cnt - 1u < d.hist_cap) {
+                d.csr_hist[cnt - 1u] = (float)p;
+            }
+        }
+        if (d.filling) {
+            d.filling[0] = f;
+            d.filling[1] = f;
+        }
+        if (d.counter) {
+            d.counter[parity ^ 1u] = cnt + 1u;
+        }
+        d.csr_valid[parity] = cnt + 1u;
+    }
+}
 
 // One row tile of one instance: phases A-H.  Every early return is uniform
 // over the workgroup.  MAXC: most mesh columns one thread handles
@@ -311,7 +356,8 @@ struct StepShared {
 // per-column register tables.  ONE_MESH trades registers for latency, as
 // suits a launch of one mesh (a batch keeps its occupancy instead): the
 // first float4 loads of the input rows are held in registers through
-// phase B, and tile 0 loads what closes the step in phase A.
+// phase B, the kernel windows load into LDS, and the projection's adds
+// keep their loads scheduled ahead.
 template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP, int MAXC,
           bool ONE_MESH>
 __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
@@ -486,25 +532,8 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             }
         }
     }
-    //  - tile 0 closes the previous step: its CSR shares, the step they
-    //    belong to (phase H sums them)
-    const bool closer = ONE_MESH && DIRECT && csr && tile == 0;
-    uint32_t cvalid_v = 0;
-    double cpv[MAXC];
-    if (closer) {
-        if (tid == 0) {
-            cvalid_v = ((const guint*)d.csr_valid)[parity ^ 1u];
-        }
-        const gdouble* shares = (const gdouble*)d.csr_part + (size_t)(parity ^ 1u) * n;
-#pragma unroll
-        for (int c = 0; c < MAXC; c++) {
-            const uint32_t k = tid + (uint32_t)c * bs;
-            cpv[c] = shares[k < d.tiles ? k : d.tiles - 1u];
-        }
-    }
     if (tid == 0) {
         cnt_sh = cnt_v;
-        sh.cvalid = cvalid_v;
     }
     if (DIRECT) {
 #pragma unroll
@@ -553,6 +582,19 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             const uint32_t y = tid + (uint32_t)c * bs;
             if (c < ncols && y < n) {
                 kick_taps<IT>(col_doff[c], n, col_didx[c], col_dw[c]);
+            }
+        }
+        // the drift reads bufB (the kicks write its rows [lr0, lr1)): in an
+        // edge tile its rows outside the mesh hold zeros, so a tap there
+        // adds +0 or -0 to a sum that starts at +0, as skipping it does,
+        // and the drift needs no per-tap check
+        if (lr0 > 0 || lr1 < R) {
+            for (uint32_t i = tid; i < (uint32_t)lr0 * n; i += bs) {
+                bufB[i] = 0.0f;
+            }
+            for (uint32_t i = (uint32_t)lr1 * n + tid; i < (uint32_t)R * n;
+                 i += bs) {
+                bufB[i] = 0.0f;
             }
         }
     }
@@ -728,8 +770,9 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             }
             const uint32_t* idx = col_didx[c];
             const float* w = col_dw[c];
-            // fast path: every tap of every output row lies in the mesh and
-            // in the buffer rows, so no tap needs a check
+            // fast path: every tap of every output row lies in the buffer
+            // rows (those outside the mesh are zeros), so no tap needs a
+            // check
             int32_t tmin = (int32_t)idx[0], tmax = (int32_t)idx[0];
 #pragma unroll
             for (int j = 1; j < IT; j++) {
@@ -738,8 +781,7 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
             }
             const int32_t xlo = x0 + tmin - half;
             const int32_t xhi = x0 + rows_out - 1 + tmax - half;
-            if (xlo >= 0 && xhi < (int32_t)n && xlo - rbase >= lr0 &&
-                xhi - rbase < lr1) {
+            if (xlo - rbase >= 0 && xhi - rbase < R) {
                 for (int orow = 0; orow < rows_out; orow++) {
                     const int32_t xb = x0 + orow - half - rbase;
                     float v = 0.0f;
@@ -878,11 +920,16 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                 for (int k = 0; k < 8; k++) {
                     qa[k] = r4[k];
                 }
+                // (one mesh: scheduling barriers keep each group of loads
+                // ahead of the adds it overlaps; merged, every 64 adds wait
+                // for a full LDS round trip.  They cost registers, which a
+                // batch keeps for its occupancy.)
                 for (; y < n; y += 64) {
 #pragma unroll
                     for (int k = 0; k < 8; k++) {
                         qb[k] = r4[((y + 32) >> 2) + k];
                     }
+                    if constexpr (ONE_MESH) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int k = 0; k < 8; k++) {
                         acc = acc + qa[k].x;
@@ -892,10 +939,12 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                     }
                     // the last pass reloads the row's first values (unused)
                     const uint32_t yn = y + 64 < n ? y + 64 : 0u;
+                    if constexpr (ONE_MESH) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int k = 0; k < 8; k++) {
                         qa[k] = r4[(yn >> 2) + k];
                     }
+                    if constexpr (ONE_MESH) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int k = 0; k < 8; k++) {
                         acc = acc + qb[k].x;
@@ -959,71 +1008,15 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
     }
     VFPS_STAMP(7);
     // ---- H: close the step (direct form) --------------------------------------
-    // Every tile leaves its CSR share; tile 0 counts the step, integrates
-    // the profile that entered it and sums the previous step's shares
-    // (written by the previous launch, so no workgroup waits for another).
+    // Every tile leaves its CSR share.  In a batch, tile 0 of each instance
+    // also closes the step (step_close); a one-mesh launch gives that to a
+    // workgroup of its own.
     if (DIRECT) {
         if (csr && tid == 0) {
             d.csr_part[(size_t)parity * n + tile] = csr_q;
         }
-        if (tile == 0) {
-            // the FP64 integral of the profile that entered the step (Simpson
-            // weights from the column registers: ws is the same on both
-            // axes) and the sum of the previous step's shares loaded in
-            // phase A, in one reduction
-            double f = 0.0, p = 0.0;
-            bool prev;
-            if constexpr (ONE_MESH) {
-#pragma unroll
-                for (int c = 0; c < MAXC; c++) {
-                    const uint32_t x = tid + (uint32_t)c * bs;
-                    if (c < ncols && x < n) {
-                        f += (double)pprev[x] * (double)col_ws[c];
-                    }
-                }
-                // uniform over the workgroup (read after phase A's barrier)
-                prev = csr && cnt >= 1 && sh.cvalid == cnt;
-                if (prev) {
-#pragma unroll
-                    for (int c = 0; c < MAXC; c++) {
-                        const uint32_t k = tid + (uint32_t)c * bs;
-                        if (k < d.tiles) {
-                            p += cpv[c];
-                        }
-                    }
-                }
-            } else {
-                // the same sums in the same order, read here
-                for (uint32_t x = tid; x < n; x += bs) {
-                    f += (double)pprev[x] * (double)ws_g[x];
-                }
-                prev = csr && cnt >= 1 &&
-                       ((const guint*)d.csr_valid)[parity ^ 1u] == cnt;
-                if (prev) {
-                    const gdouble* shares =
-                        (const gdouble*)d.csr_part + (size_t)(parity ^ 1u) * n;
-                    for (uint32_t k = tid; k < d.tiles; k += bs) {
-                        p += shares[k];
-                    }
-                }
-            }
-            block_sum2(f, p, red);
-            if (tid == 0) {
-                if (prev) {
-                    *d.csr_power = (float)p;
-                    if (d.csr_hist && cnt - 1u < d.hist_cap) {
-                        d.csr_hist[cnt - 1u] = (float)p;
-                    }
-                }
-                if (d.filling) {
-                    d.filling[0] = f;
-                    d.filling[1] = f;
-                }
-                if (d.counter) {
-                    d.counter[parity ^ 1u] = cnt + 1u;
-                }
-                d.csr_valid[parity] = cnt + 1u;
-            }
+        if (!ONE_MESH && tile == 0) {
+            step_close(d, parity, cnt, red);
         }
     }
     if (d.tstamp) {
@@ -1048,7 +1041,8 @@ enum StepForm : int {
 // The step of one launch.
 //  STEP_SINGLE: one mesh, its descriptor passed by value (the launch's own
 //  arguments, so no load of a descriptor or block map precedes the data
-//  loads), one workgroup per tile.
+//  loads), one workgroup per tile; the direct form adds a last workgroup
+//  that closes the step (step_close).
 //  Batches: blockmap[nmap] holds (instance << 16 | tile) per logical tile
 //  (STEP_NO_TILE: padding); nmap and the grid are multiples of 8.
 // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so the
@@ -1068,9 +1062,17 @@ step_kernel(const InstDesc* __restrict__ descs,
     __shared__ StepShared sh;
     const uint32_t G = gridDim.x;
     if constexpr (FORM == STEP_SINGLE) {
+        // direct form: the last workgroup closes the step, beside the tiles
+        // (in a tile it would lengthen the launch's slowest workgroup)
+        const uint32_t tiles = DIRECT ? G - 1u : G;
+        if (DIRECT && blockIdx.x == tiles) {
+            const uint32_t cnt = d0.counter ? ((const guint*)d0.counter)[parity] : 0u;
+            step_close(d0, parity, cnt, sh.red);
+            return;
+        }
         uint32_t logical = blockIdx.x;
-        if ((G & 7u) == 0u) {
-            logical = (blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+        if ((tiles & 7u) == 0u) {
+            logical = (blockIdx.x & 7u) * (tiles >> 3) + (blockIdx.x >> 3);
         }
         step_tile<IT, IPFP, DIRECT, SCHEME, CLAMP, MAXC, true>(
             d0, 0u, logical, logical, parity, lds, sh);
