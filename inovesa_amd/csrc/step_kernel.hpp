@@ -56,6 +56,9 @@ struct StepLds {
     uint32_t offw, offr;          // per-row kick offsets
     uint32_t tapw, tapr;          // per-row taps: IT shifts, then IT weights
     uint32_t csr_rows;            // FP64 CSR share of each output row
+    uint32_t fout;                // [T][n] output rows (one-mesh form),
+                                  // stored to global memory during the
+                                  // projection
     uint32_t words;
 };
 
@@ -121,6 +124,11 @@ __host__ __device__ inline StepLds step_lds(uint32_t n, uint32_t T, uint32_t str
     w += STEP_TAP_WORDS * stride;
     L.tapr = w;
     w += STEP_TAP_WORDS * stride;
+    if (windows) {
+        w = step_round4(w);
+        L.fout = w;
+        w += step_round4(T * n);
+    }
     L.words = w;
     return L;
 }
@@ -697,9 +705,6 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                 const float off = d.wake_scale * (float)acc;
                 offw[row] = off;
                 store_row_taps<IT>(off, n, tapw + row * STEP_TAP_WORDS);
-                if (row >= hlo && row < hlo + rows_out && d.wake_out) {
-                    d.wake_out[rbase + row] = off;
-                }
             } else if (job < nwk + ncsr) {
                 const int o = job - nwk;
                 csr_rows[o] = (double)pprev[x0 + o] * acc;
@@ -868,7 +873,11 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
     __syncthreads();
     VFPS_STAMP(5);
     // ---- F: Fokker-Planck along y, store, projection products ----------------
+    // (one mesh: the output rows go to LDS and out to global memory during
+    // the projection, by the threads that do not sum it: a global store
+    // here would make the summing lanes wait for its completion first)
     tile_out = L.tile_out == STEP_TILE_IN_BUF ? cur : lds + L.tile_out;
+    float* fout = lds + L.fout;
     const uint32_t tstride = n + 4;
 #pragma unroll
     for (int c = 0; c < MAXC; c++) {
@@ -888,7 +897,11 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                     v = v + src[col_fi[c][k]] * col_fw[c][k];
                 }
             }
-            out[(size_t)(x0 + orow) * n + y] = v;
+            if constexpr (ONE_MESH) {
+                fout[(size_t)orow * n + y] = v;
+            } else {
+                out[(size_t)(x0 + orow) * n + y] = v;
+            }
             tile_out[(size_t)orow * tstride + y] = v * col_ws[c];
         }
     }
@@ -904,7 +917,37 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
     const uint32_t slot = slot_sh;
     VFPS_STAMP(6);
     // ---- G: x projection ------------------------------------------------------
+    // Stores deferred to here, by threads i0, i0 + nt, ...: the wake offsets
+    // of the output rows (direct wake; issued in phase B they held up the
+    // next waits on the load counter, which drains in order) and, one mesh,
+    // the output rows.
+    auto deferred_stores = [&](uint32_t i0, uint32_t nt) {
+        if (DIRECT && wake && d.wake_out) {
+            for (uint32_t r = i0; r < (uint32_t)rows_out; r += nt) {
+                d.wake_out[x0 + (int)r] = offw[hlo + (int)r];
+            }
+        }
+        if constexpr (ONE_MESH) {
+            const uint32_t tot = (uint32_t)rows_out * n;
+            gfloat* dst = out + (size_t)x0 * n;
+            if (vec_rows) {
+                const vec4f* s4 = reinterpret_cast<const vec4f*>(fout);
+                gfloat4* d4 = (gfloat4*)dst;
+                for (uint32_t i = i0; i < (tot >> 2); i += nt) {
+                    d4[i] = s4[i];
+                }
+            } else {
+                for (uint32_t i = i0; i < tot; i += nt) {
+                    dst[i] = fout[i];
+                }
+            }
+        }
+    };
     if (d.seq_proj) {
+        // wave 0's first lanes sum; the other waves store (one wave: after)
+        if (bs > 64u && tid >= 64u) {
+            deferred_stores(tid - 64u, bs - 64u);
+        }
         // the reference's std::inner_product order: y = 0..n-1, one thread
         // per row, loads batched ahead of the dependent adds
         if ((int)tid < rows_out) {
@@ -989,7 +1032,11 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                 prof_hist[(size_t)slot * n + x] = acc;
             }
         }
+        if (bs == 64u) {
+            deferred_stores(tid, 64u);
+        }
     } else {
+        deferred_stores(tid, bs);
         for (int orow = (int)wave; orow < rows_out; orow += (int)nw) {
             const float* r = tile_out + (size_t)orow * tstride;
             float acc = 0.0f;
