@@ -819,7 +819,9 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
                         "vfps_step_create: fused step needs linear or cubic "
                         "interpolation (use fused = 0)");
         }
-        uint32_t thr = desc->threads ? desc->threads : (n >= 512 ? 512u : 256u);
+        // 512-thread workgroups from 256^2 on measured fastest on one mesh
+        // (256^2: 6.4 against 7.4 us/step with 256 threads)
+        uint32_t thr = desc->threads ? desc->threads : (n >= 256 ? 512u : 256u);
         if (ceil_div(n, thr) > (uint32_t)STEP_MAX_COLS) {
             return fail(VFPS_ERR_INVALID, "vfps_step_create: too few threads "
                                           "for the mesh size");
