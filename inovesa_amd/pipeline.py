@@ -75,7 +75,9 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=1024)
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--targets", type=int, default=8)
-    ap.add_argument("--out", default="gpurun_out/surrogate")
+    ap.add_argument("--out", default="surrogate_run",
+                    help="directory for the training data, model and scan "
+                         "summary")
     ap.add_argument("--model", default=None,
                     help="load a trained model instead of training")
     ap.add_argument("--data", default=None,
