@@ -198,9 +198,9 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is "
+        print(f"bench.py: the step runs on one GPU; --gpus {args.gpus} is "
               "not supported (independent instances shard across GPUs by "
-              "process, see the design notes)", file=sys.stderr)
+              "process: one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from inovesa_amd import Step

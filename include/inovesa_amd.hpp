@@ -47,7 +47,7 @@ typedef float meshdata_t;
 typedef float frequency_t;
 typedef float timeaxis_t;
 typedef float csrpower_t;
-typedef double integral_t;  // FP64 charge integral in this build
+typedef double integral_t;  // FP64 charge integral (the reference's is float)
 typedef std::complex<float> impedance_t;
 
 // ---- errors -------------------------------------------------------------
