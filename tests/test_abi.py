@@ -76,3 +76,65 @@ def test_initial_mesh_deterministic_and_normalised():
     _, d = ruler(64, -6, 6)
     ws = simpson_weights(64, d).astype(np.float64)
     assert abs((a.astype(np.float64) @ ws) @ ws - 1) < 1e-6
+
+
+# Hand-worked values of src/main.cpp:245-321 and inc/PS/ElectricField.hpp:81-103
+# at the defaults of src/IO/ProgramOptions.cpp (E0 1.3 GeV, sigma_delta
+# 4.7e-4, f_rev 9 MHz, H 50, V_RF 1 MV, alpha0 4e-3, PhaseSpaceSize 12,
+# I_b 3 mA, 1000 steps per synchrotron period):
+#   fs = 44.509 kHz, bl = 2.01535e-3 m
+#   dt = 1/(fs * steps) = 2.24673e-8 s
+#   revolutionpart = f_rev * dt = 0.202206
+#   bunch spacing c/(f_rev H) = 0.666205 m = 330.566 bl = 27.5472 mesh widths
+#   spacing_bins = round(N * 27.5472): 7052 (256), 14104 (512), 28208 (1024)
+#   fmax = N c / (12 bl): 3.17344e12 Hz (256), 6.34688e12 (512), 1.26938e13 (1024)
+#   wakescaling = Ib dt c / bl / (dp sigma_delta E0) with dp = 12/(N-1) and
+#   sigma_delta E0 = 611 keV: 3.48707e-4 (256), 6.98781e-4 (512),
+#   1.39893e-3 (1024)
+HAND = {
+    256: dict(spacing_bins=7052, fmax=3.17344e12, wakescaling=3.48707e-4),
+    512: dict(spacing_bins=14104, fmax=6.34688e12, wakescaling=6.98781e-4),
+    1024: dict(spacing_bins=28208, fmax=1.26938e13, wakescaling=1.39893e-3),
+}
+
+
+@pytest.mark.parametrize("n", sorted(HAND))
+def test_derived_field_constants_pinned(n):
+    """The derived constants both the library and the oracle consume: a
+    slipped formula (a missing factor, dt for revolutionpart, the wrong
+    axis length) moves one of them far beyond 1e-4."""
+    from inovesa_amd import derive
+    d = derive(grid_size=n)
+    h = HAND[n]
+    assert d["spacing_bins"] == h["spacing_bins"]
+    assert d["fmax"] == pytest.approx(h["fmax"], rel=1e-4)
+    assert d["wakescaling"] == pytest.approx(h["wakescaling"], rel=1e-4)
+    assert d["dt"] == pytest.approx(2.24673e-8, rel=1e-5)
+    assert d["revolutionpart"] == pytest.approx(0.202206, rel=1e-5)
+    assert d["steps"] == 1000
+
+
+@pytest.mark.parametrize("n", sorted(HAND))
+def test_oracle_wakescaling_pinned(orc, n):
+    """The oracle's own ElectricField scaling argument from the same ring
+    values equals the hand-worked figure, and the field divides it by nmax
+    (inc/PS/ElectricField.hpp:99-100): a constant impedance of 1 below the
+    Nyquist bin, which the reference leaves unassigned
+    (src/PS/ElectricField.cpp:229-231), turns a unit impulse at x0 into
+    wakescaling/nmax * (nmax delta(x-x0) - (-1)^(x-x0))."""
+    from inovesa_amd import derive
+    d = derive(grid_size=n)
+    dp = np.float32((d["pmax"] - d["pmin"]) / np.float32(n - 1))
+    ws = orc.wakescaling(d["Ib"], d["dt"], d["qscale"], dp, 4.7e-4, 1.3e9)
+    assert float(ws) == pytest.approx(HAND[n]["wakescaling"], rel=1e-4)
+    nmax, x0 = 64, 5
+    prof = np.zeros((1, 16), np.float32)
+    prof[0, x0] = 1.0
+    z = np.zeros(nmax, np.complex64)
+    z[: nmax // 2 + 1] = 1.0
+    wp = np.asarray(orc.wake_potential(prof, z, [0], 0, np.float32(ws))[0],
+                    np.float64).reshape(-1)
+    x = np.arange(16)
+    expect = float(ws) / nmax * (nmax * (x == x0) - (-1.0) ** (x - x0))
+    np.testing.assert_allclose(wp, expect, rtol=0,
+                               atol=2e-6 * float(ws))

@@ -14,10 +14,18 @@ def V():
     return V
 
 
-def _instances(V, sizes, per_size, kick_drift=False, seed=3):
+def _instances(V, sizes, per_size, kick_drift=False, seed=3,
+               scheme="kick_drift"):
     from inovesa_amd.workloads import (batched_scan_config, impedances,
                                        kick_drift_config, initial_mesh)
-    cfg = batched_scan_config(sizes=sizes, per_size=per_size, seed=seed)
+    if scheme == "mixed":
+        cfg = (batched_scan_config(sizes=sizes, per_size=per_size, seed=seed,
+                                   scheme="rotation")
+               + batched_scan_config(sizes=sizes, per_size=per_size,
+                                     seed=seed + 1, scheme="kick_drift"))
+    else:
+        cfg = batched_scan_config(sizes=sizes, per_size=per_size, seed=seed,
+                                  scheme=scheme)
     out = []
     for desc, d, init in cfg:
         zw, zc = impedances(d)
@@ -39,16 +47,19 @@ def _single(V, i):
     return V.Step(dict(desc, wake_method=2), init, zw, zc)
 
 
-@pytest.mark.parametrize("kick_drift", [False, True])
-def test_batch_equals_single_steps(V, kick_drift):
-    inst = _instances(V, (128, 256, 384), 2, kick_drift)
+@pytest.mark.parametrize("scheme", ["kick_drift", "rotation"])
+def test_batch_equals_single_steps(V, scheme):
+    """Each scheme alone, plus one instance of the other scheme's maps (a
+    256^2 kick-drift mesh beside the rotation instances)."""
+    inst = _instances(V, (128, 256, 384), 2, scheme == "rotation",
+                      scheme=scheme)
     b = _batch(V, inst)
     info = b.info()
     assert info["instances"] == len(inst)
     # one launch per kernel variant, workgroup size and LDS size class:
     # at least one per variant, at most one per (variant, mesh size)
-    variants = 2 if kick_drift else 1
-    assert variants <= info["launch_groups"] <= 3 + (1 if kick_drift else 0)
+    variants = 2 if scheme == "rotation" else 1
+    assert variants <= info["launch_groups"] <= 3 + (variants - 1)
     steps = 70                       # one graph replay and 6 single steps
     b.record(steps + 8)
     b.run(steps)
@@ -195,3 +206,33 @@ def test_batch_timing_and_rejects(V):
     bad = dict(inst[0][0], scheme=7)
     with pytest.raises(V.VfpsInvalid):
         V.Batch([bad], [inst[0][3]], [inst[0][1]], [inst[0][2]])
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_rotation_and_kick_drift_batch_equals_single_steps(V, persistent):
+    """The batched scan workload on the rotation maps (cubic clamped
+    RotationMap, src/SM/RotationMap.cpp:97-176) mixed with kick-drift
+    instances over 128^2-512^2, so that both kernel variants run in one
+    step, per-tile and persistent: every instance equals its run alone,
+    bit for bit."""
+    inst = _instances(V, (128, 256, 384, 512), 4, scheme="mixed")
+    assert {i[0]["scheme"] for i in inst} == {0, 1}
+    b = _batch(V, inst)
+    if persistent:
+        for k in range(len(inst)):
+            b.retile(k, 1, 256, persistent=True)
+        assert any(x["workgroups"] < x["tiles"] for x in b.launches())
+    groups = b.info()["launch_groups"]
+    assert groups >= 2
+    steps = 66
+    b.record(steps + 4)
+    b.run(steps)
+    # every mesh size of both schemes (sizes are interleaved per scheme)
+    for k in list(range(4)) + list(range(16, 20)) + [15, 31]:
+        i = inst[k]
+        s = _single(V, i)
+        s.record(steps + 4)
+        s.run(steps)
+        for x, y in zip(b.mesh(k), s.mesh()):
+            assert np.array_equal(x, y), (k, i[0]["scheme"], i[0]["n"])
+        assert np.array_equal(b.history(k)[0], s.history()[0]), k
