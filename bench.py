@@ -20,12 +20,14 @@ Further keys of the line:
                 field kernel;
   batched       64 independent meshes of 128^2-512^2 (kick-drift maps), one
                 launch per step and size class;
+  batched_rotation  the same 64 meshes on the rotation maps (wake kick,
+                cubic clamped rotation, Fokker-Planck);
   production    16 meshes of 1024^2 with the large production maps and
                 impedance;
   scan          the threshold scan with and without the trained network;
   cpu_baseline  the oracle (the FP32 C restatement of the reference CPU
-                path) on one host core, on a bounded sample of the same
-                workload.
+                path), compiled on the host with -O3 -march=native, on one
+                host core: 50 warm-up steps, then the median of 5 runs.
 """
 import argparse
 import json
@@ -41,37 +43,83 @@ METRIC = ("mesh-cell updates/s at 512²; batched+surrogate threshold scan "
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(desc, zw, zc, init, budget_s=15.0):
+def host_cpu():
+    """nproc and the lscpu model name of the host that times the baseline."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True,
+                             timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return os.cpu_count(), model
+
+
+def cpu_baseline(desc, zw, zc, init, budget_s=15.0, warmup=50, runs=5):
+    """The FP32 restatement of the reference CPU path, compiled on this host
+    with the reference's release flags (-O3 -march=native), on one thread:
+    `warmup` steps, then the median of `runs` timed runs of the same number
+    of steps (about budget_s seconds in all), each continuing the state."""
     import numpy as np
+    from oracle.build_oracle import build_native
     from oracle.oracle import Oracle
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from _helpers import oracle_cfg
-    o = Oracle()
+    o = Oracle(path=build_native())
     sim = o.sim(oracle_cfg(desc, zw.n), init, zw.data(), zc.data())
     t0 = time.perf_counter()
-    sim.run(3, want_csr=True)
-    per = (time.perf_counter() - t0) / 3
-    steps = max(3, min(2000, int(budget_s / per)))
-    t0 = time.perf_counter()
-    sim.run(steps, want_csr=True)
-    dt = time.perf_counter() - t0
+    sim.run(warmup, want_csr=True)
+    per = (time.perf_counter() - t0) / warmup
+    steps = max(10, int(budget_s / runs / per))
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        sim.run(steps, want_csr=True)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     n = desc["n"]
+    nproc, model = host_cpu()
     return {"value": n * n * steps / dt, "unit": "cell-updates/s", "cores": 1,
             "kind": "port",
-            "sample": f"oracle FP32 restatement, {steps} steps of the same "
-                      f"{n}x{n} kick-drift workload from the same initial "
-                      f"state, one thread ({dt:.1f} s)"}
+            "sample": f"oracle FP32 restatement built with -O3 -march=native "
+                      f"on this host, the same {n}x{n} kick-drift workload "
+                      f"from the same initial state, one thread: {warmup} "
+                      f"warm-up steps, then the median of {runs} runs of "
+                      f"{steps} steps ({dt:.2f} s each); FFT: the "
+                      f"restatement's own (FFTW is not available)",
+            "nproc": nproc, "cpu_model": model,
+            "run_seconds": [round(t, 3) for t in times]}
 
 
-def batched_throughput(steps, log):
-    """Independent meshes of uneven sizes (16 each of 128^2, 256^2, 384^2,
-    512^2; currents log-spaced over 0.3-3x threshold; gaps 16-64 mm;
-    rotation scheme; each its own impedance) advanced together after the
-    autotuner picked the tiles: whole-batch cell updates per second."""
+def launch_label(steps, no_graph):
+    if no_graph:
+        return "host launches"
+    full, rest = divmod(steps, 64)
+    parts = []
+    if full:
+        parts.append(f"{full} replays of a 64-step hipGraph")
+    if rest:
+        parts.append(f"one {rest}-step hipGraph")
+    return (" + ".join(parts) + " (captured before the timed region), "
+            "then one finalize launch")
+
+
+def batched_throughput(steps, log, scheme="kick_drift"):
+    """64 independent meshes of uneven sizes (16 each of 128^2, 256^2,
+    384^2, 512^2; currents log-spaced over 0.3-3x threshold; gaps 16-64 mm,
+    each its own parallel-plates impedance) advanced together after the
+    autotuner picked the tiles: whole-batch cell updates per second.
+    scheme "rotation": the maps of the 256^2 rotation workload (wake kick,
+    cubic clamped rotation, Fokker-Planck); "kick_drift": those of the
+    512^2 workload."""
     import numpy as np
     from inovesa_amd import Batch
     from inovesa_amd.workloads import batched_scan_config, impedances
-    cfg = batched_scan_config()
+    cfg = batched_scan_config(scheme=scheme)
     zs = [impedances(d) for _, d, _ in cfg]
     b = Batch([c[0] for c in cfg], [c[2] for c in cfg], [z[0] for z in zs],
               [z[1] for z in zs])
@@ -79,6 +127,7 @@ def batched_throughput(steps, log):
     cells = b.info()["cells"]
     b.record(steps + 128)
     b.run(128)
+    b.prepare(steps)
     b.dev.sync()
     t0 = time.perf_counter()
     b.run(steps)
@@ -88,12 +137,14 @@ def batched_throughput(steps, log):
                        "persistent" if r["persistent"] else "per-tile",
                        round(r["us_per_step"], 2))
               for r in table if r["chosen"]}
-    log(f"batched: {len(cfg)} instances, {cells} cells, {steps} steps in "
-        f"{dt:.2f} s")
+    log(f"batched ({scheme}): {len(cfg)} instances, {cells} cells, {steps} "
+        f"steps in {dt:.2f} s")
     return {"value": cells * steps / dt, "unit": "cell-updates/s",
-            "instances": len(cfg), "cells_per_step": int(cells),
+            "maps": scheme, "instances": len(cfg),
+            "cells_per_step": int(cells),
             "steps": steps, "ms_per_step": dt * 1e3 / steps,
             "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
+            "launch": launch_label(steps, False),
             "tiles_chosen": {str(k): v for k, v in chosen.items()}}
 
 
@@ -121,6 +172,7 @@ def production_throughput(steps, log):
     cells = b.info()["cells"]
     b.record(steps + 64)
     b.run(64)
+    b.prepare(steps)
     b.dev.sync()
     t0 = time.perf_counter()
     b.run(steps)
@@ -134,6 +186,7 @@ def production_throughput(steps, log):
             "instances": 16, "n": 1024, "cells_per_step": int(cells),
             "steps": steps, "ms_per_step": dt * 1e3 / steps,
             "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
+            "launch": launch_label(steps, False),
             "tiles_chosen": chosen[0] if chosen else None}
 
 
@@ -220,6 +273,7 @@ def main():
               init, zw, zc)
     st.record(args.warmup + args.steps, profiles=True)
     st.run(args.warmup)
+    st.prepare(args.steps)
     st.dev.sync()
     t0 = time.perf_counter()
     st.run(args.steps)
@@ -246,6 +300,7 @@ def main():
     other = Step(dict(desc, fast_projection=1 - fast,
                       no_graph=int(args.no_graph)), init, zw, zc)
     other.run(640)
+    other.prepare(min(args.steps, 12800))
     other.dev.sync()
     ko = min(args.steps, 12800)
     t0 = time.perf_counter()
@@ -261,6 +316,7 @@ def main():
     fft = Step(dict(desc, wake_method=1, fast_projection=fast,
                     no_graph=int(args.no_graph)), init, zw, zc)
     fft.run(640)
+    fft.prepare(min(args.steps, 6400))
     fft.dev.sync()
     kf = min(args.steps, 6400)
     t0 = time.perf_counter()
@@ -286,14 +342,15 @@ def main():
                       "Ib_A": d["Ib"], "wake": "direct (FP64 Toeplitz)",
                       "x_projection": ("wavefront reduction" if fast else
                                        "sequential, reference order"),
-                      "launch": "host launches" if args.no_graph else
-                                "hipGraph of 64 steps",
+                      "launch": launch_label(args.steps, args.no_graph),
                       "tile_rows": cfg["tile_rows"],
                       "threads": cfg["threads"]},
            "roofline": roofline, "fft_wake": fft_line,
            "other_projection": other_line}
     if not args.no_batch:
         out["batched"] = batched_throughput(args.batch_steps, log)
+        out["batched_rotation"] = batched_throughput(args.batch_steps, log,
+                                                     scheme="rotation")
         out["production"] = production_throughput(1000, log)
     if not args.no_scan:
         out["scan"] = scan_ablation(args.scan_targets, log)

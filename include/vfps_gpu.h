@@ -343,10 +343,16 @@ VFPS_API int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
                               const float* init, vfps_impedance* zwake,
                               vfps_impedance* zcsr, vfps_step** out);
 VFPS_API int vfps_step_destroy(vfps_step* s);
-/* Advance nsteps.  The step is captured once into a hipGraph and replayed.
- * Per-step CSR power and bunch profiles are kept on the device
- * (vfps_step_get_history). */
+/* Advance nsteps (src/main.cpp:948-1053, nsteps passes of the loop body).
+ * Unless no_graph is set, the steps replay captured hipGraphs: runs of 64
+ * steps, then one run of the remaining nsteps % 64, each captured on first
+ * use and kept per (length, starting parity).  Per-step CSR power and
+ * bunch profiles are kept on the device (vfps_step_get_history). */
 VFPS_API int vfps_step_run(vfps_step* s, uint32_t nsteps);
+/* No reference counterpart: capture, without running them, the graphs
+ * that vfps_step_run(s, nsteps) from the current state replays, so that a
+ * timed run spends no time capturing. */
+VFPS_API int vfps_step_prepare(vfps_step* s, uint32_t nsteps);
 /* record per-step outputs of the next run() calls: capacity steps of CSR
  * power (and profiles if want_profiles) */
 VFPS_API int vfps_step_record(vfps_step* s, uint32_t capacity,
@@ -392,7 +398,11 @@ VFPS_API int vfps_batch_create(vfps_dev* dev, uint32_t ninst,
                                vfps_impedance* const* zwake,
                                vfps_impedance* const* zcsr, vfps_batch** out);
 VFPS_API int vfps_batch_destroy(vfps_batch* b);
+/* nsteps steps of every instance, replayed from captured graphs as in
+ * vfps_step_run */
 VFPS_API int vfps_batch_run(vfps_batch* b, uint32_t nsteps);
+/* the graphs vfps_batch_run(b, nsteps) replays, captured ahead */
+VFPS_API int vfps_batch_prepare(vfps_batch* b, uint32_t nsteps);
 VFPS_API int vfps_batch_record(vfps_batch* b, uint32_t capacity,
                                int want_profiles);
 VFPS_API int vfps_batch_get_history(vfps_batch* b, uint32_t inst,

@@ -150,6 +150,7 @@ _SIGS = {
                                 _vp]),
     "vfps_step_destroy": (_int, [_vp]),
     "vfps_step_run": (_int, [_vp, _u32]),
+    "vfps_step_prepare": (_int, [_vp, _u32]),
     "vfps_step_record": (_int, [_vp, _u32, _int]),
     "vfps_step_get_history": (_int, [_vp, _vp, _vp, _vp]),
     "vfps_step_get_mesh": (_int, [_vp, _vp, _vp, _vp]),
@@ -161,6 +162,7 @@ _SIGS = {
                                  _vp, _vp]),
     "vfps_batch_destroy": (_int, [_vp]),
     "vfps_batch_run": (_int, [_vp, _u32]),
+    "vfps_batch_prepare": (_int, [_vp, _u32]),
     "vfps_batch_record": (_int, [_vp, _u32, _int]),
     "vfps_batch_get_history": (_int, [_vp, _u32, _vp, _vp, _vp]),
     "vfps_batch_get_mesh": (_int, [_vp, _u32, _vp, _vp, _vp]),

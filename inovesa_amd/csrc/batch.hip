@@ -58,23 +58,19 @@ struct vfps_batch {
                           // tile's LDS)
     hipEvent_t fork = nullptr;
     uint32_t parity = 0;
-    hipGraphExec_t graph = nullptr;
-    hipGraph_t graph_def = nullptr;
-    uint32_t graph_steps = 0;
+    // captured runs of k steps from parity p, key k << 1 | p
+    std::map<uint32_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 };
 
 namespace {
 
 int drop_graph(vfps_batch* b)
 {
-    if (b->graph) {
-        VFPS_HIP(hipGraphExecDestroy(b->graph));
-        b->graph = nullptr;
+    for (auto& kv : b->graphs) {
+        VFPS_HIP(hipGraphExecDestroy(kv.second.second));
+        VFPS_HIP(hipGraphDestroy(kv.second.first));
     }
-    if (b->graph_def) {
-        VFPS_HIP(hipGraphDestroy(b->graph_def));
-        b->graph_def = nullptr;
-    }
+    b->graphs.clear();
     return VFPS_OK;
 }
 
@@ -215,24 +211,46 @@ int launch_step(vfps_batch* b, uint32_t parity)
     return VFPS_OK;
 }
 
-int capture(vfps_batch* b, uint32_t steps)
+// longest captured run; a run of nsteps replays it nsteps / BATCH_GRAPH
+// times, then one captured run of the remaining steps
+constexpr uint32_t BATCH_GRAPH = 64;
+
+// the captured run of `steps` steps from parity p (captured on first use)
+int batch_graph(vfps_batch* b, uint32_t steps, uint32_t p, hipGraphExec_t* out)
 {
-    int rc = drop_graph(b);
-    if (rc != VFPS_OK) return rc;
+    const uint32_t key = steps << 1 | (p & 1u);
+    auto it = b->graphs.find(key);
+    if (it != b->graphs.end()) {
+        *out = it->second.second;
+        return VFPS_OK;
+    }
+    if (b->graphs.size() >= 16) {
+        int rc = drop_graph(b);
+        if (rc != VFPS_OK) return rc;
+    }
     hipStream_t st = b->dev->stream;
     VFPS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (uint32_t k = 0; k < steps; k++) {
-        rc = launch_step(b, k & 1u);
+        int rc = launch_step(b, (p + k) & 1u);
         if (rc != VFPS_OK) {
-            hipGraph_t g;
+            hipGraph_t g = nullptr;
             hipStreamEndCapture(st, &g);
             if (g) hipGraphDestroy(g);
             return rc;
         }
     }
-    VFPS_HIP(hipStreamEndCapture(st, &b->graph_def));
-    VFPS_HIP(hipGraphInstantiate(&b->graph, b->graph_def, nullptr, nullptr, 0));
-    b->graph_steps = steps;
+    hipGraph_t def = nullptr;
+    hipGraphExec_t exec = nullptr;
+    VFPS_HIP(hipStreamEndCapture(st, &def));
+    hipError_t e = hipGraphInstantiate(&exec, def, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(def);
+        return fail(VFPS_ERR_HIP, hipGetErrorString(e));
+    }
+    b->graphs[key] = {def, exec};
+    // uploaded now, so that the first replay does not pay for it
+    VFPS_HIP(hipGraphUpload(exec, st));
+    *out = exec;
     return VFPS_OK;
 }
 
@@ -308,25 +326,15 @@ int vfps_batch_run(vfps_batch* b, uint32_t nsteps)
     hipStream_t st = b->dev->stream;
     uint32_t left = nsteps;
     int rc;
-    if ((b->parity & 1u) && left > 0) {
-        if ((rc = launch_step(b, 1)) != VFPS_OK) return rc;
-        b->parity = 0;
-        left--;
-    }
-    const uint32_t G = 64;
-    if (left >= G) {
-        if (!b->graph || b->graph_steps != G) {
-            if ((rc = capture(b, G)) != VFPS_OK) return rc;
-        }
-        while (left >= G) {
-            VFPS_HIP(hipGraphLaunch(b->graph, st));
-            left -= G;
-        }
-    }
+    // captured runs: full ones of BATCH_GRAPH steps (even, so the parity
+    // stays), then one of the remaining steps
     while (left > 0) {
-        if ((rc = launch_step(b, b->parity)) != VFPS_OK) return rc;
-        b->parity ^= 1u;
-        left--;
+        const uint32_t k = left >= BATCH_GRAPH ? BATCH_GRAPH : left;
+        hipGraphExec_t g = nullptr;
+        if ((rc = batch_graph(b, k, b->parity, &g)) != VFPS_OK) return rc;
+        VFPS_HIP(hipGraphLaunch(g, st));
+        left -= k;
+        b->parity ^= (k & 1u);
     }
     if (nsteps > 0) {
         for (auto& g : b->groups) {
@@ -346,6 +354,25 @@ int vfps_batch_run(vfps_batch* b, uint32_t nsteps)
         }
     }
     return VFPS_OK;
+}
+
+int vfps_batch_prepare(vfps_batch* b, uint32_t nsteps)
+{
+    VFPS_CHECK_ARG(b, "vfps_batch_prepare: NULL batch");
+    VFPS_TRY_BEGIN
+    hipGraphExec_t g = nullptr;
+    int rc;
+    if (nsteps >= BATCH_GRAPH &&
+        (rc = batch_graph(b, BATCH_GRAPH, b->parity, &g)) != VFPS_OK) {
+        return rc;
+    }
+    if (nsteps % BATCH_GRAPH &&
+        (rc = batch_graph(b, nsteps % BATCH_GRAPH, b->parity, &g)) != VFPS_OK) {
+        return rc;
+    }
+    VFPS_HIP(hipStreamSynchronize(b->dev->stream));
+    return VFPS_OK;
+    VFPS_TRY_END
 }
 
 int vfps_batch_record(vfps_batch* b, uint32_t capacity, int want_profiles)

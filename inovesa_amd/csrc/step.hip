@@ -391,8 +391,11 @@ static void step_free(vfps_step* s)
 {
     if (!s) return;
     if (s->dev) hipStreamSynchronize(s->dev->stream);
-    if (s->graph) hipGraphExecDestroy(s->graph);
-    if (s->graph_def) hipGraphDestroy(s->graph_def);
+    for (auto& kv : s->graphs) {
+        hipGraphExecDestroy(kv.second.second);
+        hipGraphDestroy(kv.second.first);
+    }
+    s->graphs.clear();
     vfps_map_destroy(s->wm);
     vfps_map_destroy(s->rfm);
     vfps_map_destroy(s->drm);
@@ -533,35 +536,56 @@ static int step_flush_desc(vfps_step* s)
 
 static int step_drop_graph(vfps_step* s)
 {
-    if (s->graph) {
-        VFPS_HIP(hipGraphExecDestroy(s->graph));
-        s->graph = nullptr;
+    for (auto& kv : s->graphs) {
+        VFPS_HIP(hipGraphExecDestroy(kv.second.second));
+        VFPS_HIP(hipGraphDestroy(kv.second.first));
     }
-    if (s->graph_def) {
-        VFPS_HIP(hipGraphDestroy(s->graph_def));
-        s->graph_def = nullptr;
-    }
+    s->graphs.clear();
     return VFPS_OK;
 }
 
-static int step_capture(vfps_step* s, uint32_t steps)
+// longest captured run; a run of nsteps replays it nsteps / STEP_GRAPH
+// times, then one captured run of the remaining steps
+constexpr uint32_t STEP_GRAPH = 64;
+
+// the captured run of `steps` steps from parity p (captured on first use)
+static int step_graph(vfps_step* s, uint32_t steps, uint32_t p,
+                      hipGraphExec_t* out)
 {
-    int rc = step_drop_graph(s);
-    if (rc != VFPS_OK) return rc;
+    const uint32_t key = steps << 1 | (p & 1u);
+    auto it = s->graphs.find(key);
+    if (it != s->graphs.end()) {
+        *out = it->second.second;
+        return VFPS_OK;
+    }
+    // remainders differ from run to run: keep the cache small
+    if (s->graphs.size() >= 16) {
+        int rc = step_drop_graph(s);
+        if (rc != VFPS_OK) return rc;
+    }
     hipStream_t st = s->dev->stream;
     VFPS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (uint32_t k = 0; k < steps; k++) {
-        rc = step_launch_one(s, k & 1u);
+        int rc = step_launch_one(s, s->fused ? ((p + k) & 1u) : 0u);
         if (rc != VFPS_OK) {
-            hipGraph_t g;
+            hipGraph_t g = nullptr;
             hipStreamEndCapture(st, &g);
             if (g) hipGraphDestroy(g);
             return rc;
         }
     }
-    VFPS_HIP(hipStreamEndCapture(st, &s->graph_def));
-    VFPS_HIP(hipGraphInstantiate(&s->graph, s->graph_def, nullptr, nullptr, 0));
-    s->graph_steps = steps;
+    hipGraph_t def = nullptr;
+    hipGraphExec_t exec = nullptr;
+    VFPS_HIP(hipStreamEndCapture(st, &def));
+    hipError_t e = hipGraphInstantiate(&exec, def, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(def);
+        return fail(VFPS_ERR_HIP, hipGetErrorString(e));
+    }
+    s->graphs[key] = {def, exec};
+    // uploaded now, so that the first replay does not pay for it
+    VFPS_HIP(hipGraphUpload(exec, st));
+    *out = exec;
     return VFPS_OK;
 }
 
@@ -883,20 +907,17 @@ int vfps_step_run(vfps_step* s, uint32_t nsteps)
     uint32_t left = nsteps;
     int rc = step_flush_desc(s);
     if (rc != VFPS_OK) return rc;
-    // bring the fused ping-pong to parity 0 before replaying the graph
-    if (s->fused && (s->parity & 1u) && left > 0) {
-        if ((rc = step_launch_one(s, 1)) != VFPS_OK) return rc;
-        s->parity = 0;
-        left--;
-    }
-    const uint32_t G = 64;
-    if (left >= G && !s->desc.no_graph) {
-        if (!s->graph || s->graph_steps != G) {
-            if ((rc = step_capture(s, G)) != VFPS_OK) return rc;
-        }
-        while (left >= G) {
-            VFPS_HIP(hipGraphLaunch(s->graph, st));
-            left -= G;
+    if (!s->desc.no_graph) {
+        // captured runs: full ones of STEP_GRAPH steps (even, so the parity
+        // stays), then one of the remaining steps
+        const uint32_t p = s->fused ? (s->parity & 1u) : 0u;
+        while (left > 0) {
+            const uint32_t k = left >= STEP_GRAPH ? STEP_GRAPH : left;
+            hipGraphExec_t g = nullptr;
+            if ((rc = step_graph(s, k, p, &g)) != VFPS_OK) return rc;
+            VFPS_HIP(hipGraphLaunch(g, st));
+            left -= k;
+            if (s->fused && (k & 1u)) s->parity ^= 1u;
         }
     }
     while (left > 0) {
@@ -916,6 +937,28 @@ int vfps_step_run(vfps_step* s, uint32_t nsteps)
         return fail(VFPS_ERR_INVALID, "fused step: a tap fell outside the "
                                       "halo rows (internal error)");
     }
+    return VFPS_OK;
+}
+
+int vfps_step_prepare(vfps_step* s, uint32_t nsteps)
+{
+    VFPS_CHECK_ARG(s, "vfps_step_prepare: NULL step");
+    int rc = step_flush_desc(s);
+    if (rc != VFPS_OK) return rc;
+    if (s->desc.no_graph || nsteps == 0) {
+        return VFPS_OK;
+    }
+    const uint32_t p = s->fused ? (s->parity & 1u) : 0u;
+    hipGraphExec_t g = nullptr;
+    if (nsteps >= STEP_GRAPH &&
+        (rc = step_graph(s, STEP_GRAPH, p, &g)) != VFPS_OK) {
+        return rc;
+    }
+    if (nsteps % STEP_GRAPH &&
+        (rc = step_graph(s, nsteps % STEP_GRAPH, p, &g)) != VFPS_OK) {
+        return rc;
+    }
+    VFPS_HIP(hipStreamSynchronize(s->dev->stream));
     return VFPS_OK;
 }
 

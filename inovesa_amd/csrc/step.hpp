@@ -1,6 +1,8 @@
 // Fused per-step kernels shared by the single-mesh step and the batch.
 #pragma once
 
+#include <map>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -143,9 +145,8 @@ struct vfps_step {
     bool fused = true;
     bool direct = true;
     uint32_t parity = 0;  // fused path: buffer holding the state (0: t1)
-    hipGraphExec_t graph = nullptr;
-    hipGraph_t graph_def = nullptr;
-    uint32_t graph_steps = 0;
+    // captured runs of k steps from parity p, key k << 1 | p
+    std::map<uint32_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 
     uint32_t tiles() const { return vfps_impl::ceil_div(desc.n, T); }
 };

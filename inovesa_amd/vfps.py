@@ -556,6 +556,11 @@ class Step:
     def run(self, nsteps):
         call("vfps_step_run", self.h, nsteps)
 
+    def prepare(self, nsteps):
+        """Capture the graphs a run of nsteps from the current state replays
+        (a timed run then spends no time capturing)."""
+        call("vfps_step_prepare", self.h, nsteps)
+
     def record(self, capacity, profiles=False):
         call("vfps_step_record", self.h, capacity, int(profiles))
         self.cap = capacity
@@ -645,6 +650,10 @@ class Batch:
 
     def run(self, nsteps):
         call("vfps_batch_run", self.h, nsteps)
+
+    def prepare(self, nsteps):
+        """Capture the graphs a run of nsteps from the current state replays."""
+        call("vfps_batch_prepare", self.h, nsteps)
 
     def record(self, capacity, profiles=False):
         call("vfps_batch_record", self.h, capacity, int(profiles))

@@ -37,5 +37,24 @@ def build(verbose=False):
     return outs
 
 
+def build_native(outdir=None, verbose=False):
+    """The FP32 restatement built with the reference's release flags
+    (CMakeLists.txt:22,164: -O3 -march=native, the compiler's default
+    contraction of a*b+c) on the machine that times it: the CPU baseline of
+    the benchmark.  Not the parity target (that one keeps every rounding,
+    -ffp-contract=off).  Returns the library path."""
+    outdir = outdir or os.path.join(HERE, "_native")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "liboracle_f32_native.so")
+    tmp = out + ".tmp%d" % os.getpid()
+    cmd = ["gcc", "-O3", "-march=native", "-fPIC", "-shared", "-std=gnu11",
+           "-w", SRC, "-o", tmp, "-lm"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)

@@ -29,9 +29,11 @@ class SimCfg(C.Structure):
 
 
 class Oracle:
-    def __init__(self, fp64=False):
+    def __init__(self, fp64=False, path=None):
+        """path: another build of the same source (the natively compiled
+        CPU baseline of the benchmark)."""
         name = "liboracle_f64.so" if fp64 else "liboracle_f32.so"
-        path = os.path.join(HERE, name)
+        path = path or os.path.join(HERE, name)
         if not os.path.exists(path):
             from oracle.build_oracle import build
             build()
