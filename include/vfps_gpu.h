@@ -230,6 +230,26 @@ VFPS_API int vfps_field_get_padded(vfps_field* f, float* out);
 /* wakePotential (src/PS/ElectricField.cpp:186-253); wp ([nb][n]) may be
  * NULL (the result stays on the device); non-NULL synchronises */
 VFPS_API int vfps_field_wake_potential(vfps_field* f, float* wp);
+/* No reference counterpart.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): vfps_field_wake_potential without a field object, on
+ * caller-owned device buffers.  proj: device [nb][n] x projections; z:
+ * device impedance table of nmax entries; buckets: host array of nb bucket
+ * numbers (profile b sits at buckets[b] * spacing_bins of the padded
+ * array); wakescaling: the ElectricField constructor argument (divided by
+ * nmax here, as there); wake: device [nb][n] output.  The library allocates
+ * nothing: its scratch (FFT twiddle tables, bucket numbers) is a fixed
+ * device array of 128.25 KiB per GPU, held until the process ends, so
+ * nmax <= 8192 and nb <= 64 (larger: VFPS_ERR_INVALID).  Asynchronous on
+ * the device stream.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other threads, other device handles of the same GPU)
+ * race. */
+VFPS_API int vfps_wake_potential_noalloc(vfps_dev* dev, const float* proj,
+                                         uint32_t nb, uint32_t n,
+                                         const vfps_cplx* z, uint32_t nmax,
+                                         const uint32_t* buckets,
+                                         uint32_t spacing_bins,
+                                         float wakescaling, float* wake);
 /* getWakeScaling (inc/PS/ElectricField.hpp:306-307) */
 VFPS_API int vfps_field_get_wakescaling(const vfps_field* f, float* s);
 /* updateCSR(fc) (src/PS/ElectricField.cpp:133-184); spectrum ([nb][nmax])

@@ -136,6 +136,8 @@ _SIGS = {
     "vfps_impedance_download": (_int, [_vp, _vp]),
     "vfps_impedance_nfreqs": (_int, [_vp, _vp]),
     "vfps_airy": (_int, [_vp, _vp, _u32, _vp, _vp]),
+    "vfps_wake_potential_noalloc": (_int, [_vp, _vp, _u32, _u32, _vp, _u32,
+                                           _vp, _u32, C.c_float, _vp]),
     "vfps_field_create": (_int, [_vp, _vp, _vp, _u32, _u32, _f64, _f32, _f32,
                                  _vp]),
     "vfps_field_wakescaling": (_f32, [_f64, _f64, _f64, _f32, _f64, _f64]),

@@ -6,6 +6,8 @@
 // multiply by Z, the Hermitian post/pre passes, the scaling and the CSR
 // spectrum are fused into the pass between the two FFTs.
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "common.hpp"
 #include "field_body.hpp"
@@ -125,11 +127,133 @@ int launch_wake_potential(vfps_field* f, float* dst)
     return launch_field(p, true, false, 1, f->dev->stream);
 }
 
+// ---- the wake potential without allocation ---------------------------------
+// Scratch of vfps_wake_potential_noalloc: the FFT twiddle tables and the
+// bucket numbers, in a file-scope device array (one copy per GPU, held until
+// the process ends).  Host code never uses the array's name as an address:
+// its device address is taken once per device (noalloc_scratch) and that
+// pointer goes to the kernels.
+constexpr uint32_t NOALLOC_MAX_NMAX = 8192;
+constexpr uint32_t NOALLOC_MAX_BUNCHES = 64;
+constexpr size_t NOALLOC_TW_BYTES = sizeof(double2) * (NOALLOC_MAX_NMAX / 2);
+constexpr size_t NOALLOC_BYTES =
+    2 * NOALLOC_TW_BYTES + sizeof(uint32_t) * NOALLOC_MAX_BUNCHES;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static void* noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+// the device address of `scratch` on dev (taken on the first call for that
+// device, with that device current)
+static int noalloc_scratch(const vfps_dev* dev, unsigned char** out)
+{
+    VFPS_CHECK_ARG(dev->ordinal >= 0 && dev->ordinal < NOALLOC_MAX_DEVICES,
+                   "vfps_wake_potential_noalloc: device ordinal");
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev->ordinal]) {
+        int cur = 0;
+        VFPS_HIP(hipGetDevice(&cur));
+        if (cur != dev->ordinal) VFPS_HIP(hipSetDevice(dev->ordinal));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev->ordinal) hipSetDevice(cur);
+        if (e != hipSuccess || !ptr) {
+            return fail(VFPS_ERR_HIP, std::string("hipGetSymbolAddress: ") +
+                                          hipGetErrorString(e));
+        }
+        noalloc_ptr[dev->ordinal] = ptr;
+    }
+    *out = static_cast<unsigned char*>(noalloc_ptr[dev->ordinal]);
+    return VFPS_OK;
+}
+
+struct NoAllocBuckets {
+    uint32_t b[NOALLOC_MAX_BUNCHES];
+};
+
+// the twiddle tables of make_twiddles and the bucket numbers, into the
+// scratch (every entry the field kernel reads is written: nothing carries
+// over from an earlier call)
+__global__ void noalloc_setup_kernel(uint32_t M, double2* tw_m, double2* tw_n,
+                                     uint32_t nb, NoAllocBuckets bk,
+                                     uint32_t* buckets)
+{
+    const double pi = 3.141592653589793238462643383279502884;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < M;
+         t += gridDim.x * blockDim.x) {
+        const double a = -2.0 * pi * (double)t / (double)M;
+        const double b = -2.0 * pi * (double)t / (2.0 * (double)M);
+        tw_m[t] = make_double2(cos(a), sin(a));
+        tw_n[t] = make_double2(cos(b), sin(b));
+    }
+    if (blockIdx.x == 0 && threadIdx.x < nb) {
+        buckets[threadIdx.x] = bk.b[threadIdx.x];
+    }
+}
+
 }  // namespace vfps_impl
 
 using namespace vfps_impl;
 
 extern "C" {
+
+int vfps_wake_potential_noalloc(vfps_dev* dev, const float* proj, uint32_t nb,
+                                uint32_t n, const vfps_cplx* z, uint32_t nmax,
+                                const uint32_t* buckets, uint32_t spacing_bins,
+                                float wakescaling, float* wake)
+{
+    VFPS_CHECK_ARG(dev && proj && z && buckets && wake,
+                   "vfps_wake_potential_noalloc: NULL argument");
+    VFPS_CHECK_ARG(nb >= 1 && nb <= NOALLOC_MAX_BUNCHES && n >= 1,
+                   "vfps_wake_potential_noalloc: 1 to 64 bunches of n >= 1 "
+                   "bins");
+    VFPS_CHECK_ARG(nmax >= 4 && (nmax & (nmax - 1)) == 0,
+                   "vfps_wake_potential_noalloc: FFT length must be a power "
+                   "of two >= 4");
+    VFPS_CHECK_ARG(nmax <= NOALLOC_MAX_NMAX,
+                   "vfps_wake_potential_noalloc: FFT length exceeds the "
+                   "scratch maximum of 8192");
+    for (uint32_t b = 0; b < nb; b++) {
+        VFPS_CHECK_ARG((size_t)buckets[b] * spacing_bins + n <= nmax,
+                       "vfps_wake_potential_noalloc: bunch does not fit the "
+                       "padded array");
+    }
+    const uint32_t M = nmax / 2;
+    VFPS_CHECK_ARG(field_lds_bytes(M) <= dev->lds_max,
+                   "vfps_wake_potential_noalloc: FFT length exceeds the LDS "
+                   "of one CU");
+    VFPS_TRY_BEGIN
+    unsigned char* base = nullptr;
+    int rc = noalloc_scratch(dev, &base);
+    if (rc != VFPS_OK) return rc;
+    if ((rc = field_set_lds_limit(field_lds_bytes(M))) != VFPS_OK) return rc;
+    double2* tw_m = reinterpret_cast<double2*>(base);
+    double2* tw_n = reinterpret_cast<double2*>(base + NOALLOC_TW_BYTES);
+    uint32_t* bk_d = reinterpret_cast<uint32_t*>(base + 2 * NOALLOC_TW_BYTES);
+    NoAllocBuckets bk{};
+    for (uint32_t b = 0; b < nb; b++) bk.b[b] = buckets[b];
+    hipStream_t st = dev->stream;
+    hipLaunchKernelGGL(noalloc_setup_kernel, dim3(ceil_div(M, 256u)),
+                       dim3(256), 0, st, M, tw_m, tw_n, nb, bk, bk_d);
+    VFPS_HIP(hipGetLastError());
+    FieldParams p{};
+    p.nb = nb;
+    p.n = n;
+    p.M = M;
+    p.spacing = spacing_bins;
+    p.buckets = bk_d;
+    // _wakescaling(wakescaling/_nmax) (src/PS/ElectricField.cpp:58)
+    p.scaling = wakescaling / (float)nmax;
+    p.proj = proj;
+    p.Z = reinterpret_cast<const float2*>(z);
+    p.tw_m = tw_m;
+    p.tw_n = tw_n;
+    p.wake = wake;
+    return launch_field(p, true, false, 1, st);
+    VFPS_TRY_END
+}
 
 float vfps_field_wakescaling(double Ib, double dt, double qscale,
                              float delta1, double sigma_delta, double E0)

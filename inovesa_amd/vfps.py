@@ -20,7 +20,7 @@ __all__ = ["Device", "PhaseSpace", "KickMap", "DriftMap", "RFKickMap",
            "ElectricField", "Step", "derive", "airy", "VfpsError",
            "VfpsInvalid", "InterpolationType", "Axis", "FPType",
            "DerivationType", "toeplitz_kernels", "Batch", "MLP",
-           "gemm_bf16", "round_bf16"]
+           "gemm_bf16", "round_bf16", "wake_potential_noalloc"]
 
 
 class InterpolationType:  # inc/SM/SourceMap.hpp:22-27
@@ -421,6 +421,19 @@ def makeImpedance(nfreqs, fmax, R_bend, frev, gap, use_csr=True, s=0.0,
         changed = True
         rv += Impedance(nfreqs, fmax, dev, table=table)
     return rv if changed else None
+
+
+def wake_potential_noalloc(proj_ptr, nb, n, z_ptr, nmax, buckets,
+                           spacing_bins, wakescaling, wake_ptr, dev=None):
+    """The wake potential of device profiles (proj_ptr: [nb][n] float32)
+    with the device impedance z_ptr (nmax complex64) into wake_ptr ([nb][n]
+    float32), allocating nothing (vfps_wake_potential_noalloc).  Pointers
+    are device addresses (integers); asynchronous on the device stream."""
+    dev = dev or Device.default()
+    b = np.ascontiguousarray(buckets, np.uint32)
+    call("vfps_wake_potential_noalloc", dev.h, C.c_void_p(proj_ptr), nb, n,
+         C.c_void_p(z_ptr), nmax, _p(b), spacing_bins, wakescaling,
+         C.c_void_p(wake_ptr))
 
 
 class ElectricField:
