@@ -544,6 +544,60 @@ public:
     }
 };
 
+// DynamicRFKickMap (src/SM/DynamicRFKickMap.cpp:15-67): the reference's two
+// constructors plus the noise seed, which the reference draws from
+// std::random_device.  getPastModulation() returns the (phase, amplitude)
+// pairs applied since the last call.
+class DynamicRFKickMap : public KickMap {
+public:
+    // linear RF
+    DynamicRFKickMap(std::shared_ptr<PhaseSpace> in,
+                     std::shared_ptr<PhaseSpace> out, meshindex_t, meshindex_t,
+                     meshaxis_t angle, double revolutionpart, double f_RF,
+                     meshaxis_t phasespread, meshaxis_t amplspread,
+                     meshaxis_t modampl, double modtimeincrement,
+                     uint32_t steps, InterpolationType it, bool interpol_clamp,
+                     oclhptr_t, uint32_t seed = 0)
+        : KickMap(std::move(in), std::move(out))
+    {
+        check(vfps_map_create_dynrfkick_linear(
+            in_->handle(), out_->handle(), angle, revolutionpart,
+            (float)f_RF, phasespread, amplspread, modampl, modtimeincrement,
+            steps, seed, it, interpol_clamp ? 1 : 0, &h_));
+    }
+    // sinusoidal RF
+    DynamicRFKickMap(std::shared_ptr<PhaseSpace> in,
+                     std::shared_ptr<PhaseSpace> out, meshindex_t, meshindex_t,
+                     double revolutionpart, double V_RF, double f_RF, double V0,
+                     meshaxis_t phasespread, meshaxis_t amplspread,
+                     meshaxis_t modampl, double modtimeincrement,
+                     uint32_t steps, InterpolationType it, bool interpol_clamp,
+                     oclhptr_t, uint32_t seed = 0)
+        : KickMap(std::move(in), std::move(out))
+    {
+        check(vfps_map_create_dynrfkick(
+            in_->handle(), out_->handle(), revolutionpart, (float)V_RF,
+            (float)f_RF, (float)V0, phasespread, amplspread, modampl,
+            modtimeincrement, steps, seed, it, interpol_clamp ? 1 : 0, &h_));
+    }
+
+    std::vector<std::array<meshaxis_t, 2>> getPastModulation()
+    {
+        uint32_t applied = 0;
+        check(vfps_map_get_modulation(h_, 0, 0, nullptr, &applied));
+        std::vector<std::array<meshaxis_t, 2>> rv(applied - read_);
+        if (!rv.empty()) {
+            check(vfps_map_get_modulation(h_, read_, applied - read_,
+                                          rv[0].data(), nullptr));
+        }
+        read_ = applied;
+        return rv;
+    }
+
+private:
+    uint32_t read_ = 0;
+};
+
 // RotationMap(in, out, xsize, ysize, angle, it, clamped, rotmapsize, oclh)
 // (src/SM/RotationMap.cpp:12-95)
 class RotationMap : public SourceMap {

@@ -140,6 +140,48 @@ VFPS_API int vfps_map_create_rfkick(vfps_mesh* in, vfps_mesh* out,
                                     float revolutionpart, float V_RF,
                                     float f_RF, float V0, int interp,
                                     int clamp, vfps_map** map);
+/* DynamicRFKickMap, sinusoidal (src/SM/DynamicRFKickMap.cpp:45-67):
+ * an RFKickMap whose kick is recomputed before every apply() from the next
+ * entry of a modulation queue of `steps` (phase, amplitude) pairs built at
+ * construction (__calcModulation, :80-95: normally distributed phase and
+ * amplitude noise of phasespread/sqrt(revolutionpart) and
+ * amplspread/sqrt(revolutionpart), plus a phase modulation
+ * modampl*sin(2 pi modtimeincrement i)).  The noise comes from
+ * std::mt19937 seeded with `seed` (the reference seeds it from
+ * std::random_device) and std::normal_distribution<float>.
+ * revolutionpart is the reference's double (the kick holds it as float).
+ * An apply() beyond `steps` fails with VFPS_ERR_INVALID. */
+VFPS_API int vfps_map_create_dynrfkick(vfps_mesh* in, vfps_mesh* out,
+                                       double revolutionpart, float V_RF,
+                                       float f_RF, float V0,
+                                       float phasespread, float amplspread,
+                                       float modampl, double modtimeincrement,
+                                       uint32_t steps, uint32_t seed,
+                                       int interp, int clamp, vfps_map** map);
+/* DynamicRFKickMap, linear (src/SM/DynamicRFKickMap.cpp:15-38): the
+ * linear RF kick of src/SM/RFKickMap.cpp:59-69 with the same modulation
+ * (synchronous phase 0) */
+VFPS_API int vfps_map_create_dynrfkick_linear(vfps_mesh* in, vfps_mesh* out,
+                                              float angle,
+                                              double revolutionpart,
+                                              float f_RF, float phasespread,
+                                              float amplspread, float modampl,
+                                              double modtimeincrement,
+                                              uint32_t steps, uint32_t seed,
+                                              int interp, int clamp,
+                                              vfps_map** map);
+/* DynamicRFKickMap::getPastModulation (src/SM/DynamicRFKickMap.cpp:113-119):
+ * entries [first, first+count) of the modulation queue ([count][2]: phase,
+ * amplitude) and the number applied so far */
+VFPS_API int vfps_map_get_modulation(const vfps_map* map, uint32_t first,
+                                     uint32_t count, float* out,
+                                     uint32_t* applied);
+/* the modulation queue alone (host only, no device): [steps][2] */
+VFPS_API int vfps_dynrf_modulation(uint32_t steps, uint32_t seed,
+                                   float syncphase, double revolutionpart,
+                                   float phasespread, float amplspread,
+                                   float modampl, double modtimeincrement,
+                                   float* out);
 /* RotationMap(in,out,xsize,ysize,angle,it,clamped,rotmapsize,oclh)
  * (src/SM/RotationMap.cpp:12-95).  The taps are computed on the fly for
  * any rotmapsize; rotmapsize only selects the reference semantics (clamping

@@ -114,6 +114,19 @@ _SIGS = {
                                              _vp]),
     "vfps_map_create_rfkick": (_int, [_vp, _vp, _f32, _f32, _f32, _f32, _int,
                                       _int, _vp]),
+    "vfps_map_create_dynrfkick": (_int, [_vp, _vp, C.c_double, C.c_float,
+                                         C.c_float, C.c_float, C.c_float,
+                                         C.c_float, C.c_float, C.c_double,
+                                         _u32, _u32, _int, _int, _vp]),
+    "vfps_map_create_dynrfkick_linear": (_int, [_vp, _vp, C.c_float,
+                                                C.c_double, C.c_float,
+                                                C.c_float, C.c_float,
+                                                C.c_float, C.c_double, _u32,
+                                                _u32, _int, _int, _vp]),
+    "vfps_map_get_modulation": (_int, [_vp, _u32, _u32, _vp, _vp]),
+    "vfps_dynrf_modulation": (_int, [_u32, _u32, C.c_float, C.c_double,
+                                     C.c_float, C.c_float, C.c_float,
+                                     C.c_double, _vp]),
     "vfps_map_create_rotation": (_int, [_vp, _vp, _f32, _int, _int, _u32,
                                         _vp]),
     "vfps_map_create_fokkerplanck": (_int, [_vp, _vp, _int, _int, _f32, _int,

@@ -343,11 +343,15 @@ int vfps_batch_run(vfps_batch* b, uint32_t nsteps)
             if (rc != VFPS_OK) return rc;
         }
     }
+    // every member's error flag comes back in the same stream: one wait
+    for (auto* s : b->members) {
+        VFPS_HIP(hipMemcpyAsync(s->err_h, s->err, sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, st));
+    }
     VFPS_HIP(hipStreamSynchronize(st));
     sync_parity(b);
     for (auto* s : b->members) {
-        uint32_t err = 0;
-        VFPS_HIP(hipMemcpy(&err, s->err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        const uint32_t err = *s->err_h;
         if (err) {
             return fail(VFPS_ERR_INVALID, "batch step: a tap fell outside the "
                                           "halo rows (internal error)");

@@ -111,6 +111,17 @@ struct vfps_map {
     int ip = 0;
     float* fp_w = nullptr;      // [ip][n]
     uint32_t* fp_i = nullptr;   // [ip][n]
+    // DynamicRFKickMap: RF kick offsets recomputed before every apply from
+    // the next (phase, amplitude) of a modulation queue built at
+    // construction
+    bool dynrf = false;
+    bool rf_linear = false;
+    float rf_angle = 0, rf_revolutionpart = 0, rf_V = 0, rf_f = 0, rf_V0 = 0;
+    float rf_syncphase = 0;
+    std::vector<float> modulation;  // [steps][2]: phase, amplitude
+    uint32_t mod_next = 0;          // entries applied so far
+    float* stage_h = nullptr;       // pinned host offsets of the next apply
+    hipEvent_t stage_done = nullptr;  // the last upload from stage_h
 };
 
 struct vfps_impedance {

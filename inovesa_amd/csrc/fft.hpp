@@ -126,6 +126,44 @@ __device__ void fft_lds(double2* a, uint32_t M, const double2* tw)
     }
 }
 
+// Complex DFT of length M (any M) in place in the LDS buffer `a`, by the
+// direct sum out[k] = sum_j a[j] w^(j k) in FP64, for the lengths the
+// radix-4/2 FFT does not take (the reference's FFTW plans any length, e.g.
+// nmax = 768 = 3 * 256 when RoundPadding is off).  tmp: LDS scratch of M
+// entries; tw: the full table exp(-2 pi i t / M), t < M (global memory: the
+// powers of w are picked by index, (j k) mod M, so no rounding accumulates
+// across the sum).  Forward uses exp(-i..), INV exp(+i..); unnormalised.
+// Must be called by the whole workgroup.
+template <bool INV>
+__device__ void dft_lds(double2* a, double2* tmp, uint32_t M,
+                        const double2* __restrict__ tw)
+{
+    for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
+        tmp[i] = a[i];
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < M; k += blockDim.x) {
+        double re = 0.0, im = 0.0;
+        uint32_t t = 0;
+        for (uint32_t j = 0; j < M; j++) {
+            const double2 x = tmp[j];
+            const double2 w = tw[t];
+            const double wy = INV ? -w.y : w.y;
+            re += x.x * w.x - x.y * wy;
+            im += x.x * wy + x.y * w.x;
+            t += k;
+            if (t >= M) t -= M;
+        }
+        a[k] = make_double2(re, im);
+    }
+    __syncthreads();
+}
+
+__host__ __device__ __forceinline__ bool fft_pow2(uint32_t M)
+{
+    return M >= 1u && (M & (M - 1u)) == 0u;
+}
+
 // Spectrum F_k (k = 0..M) of the real input from Z = FFT_M(z):
 // for the pair (k, M-k), 0 < k <= M/2.  wk = exp(-2 pi i k / N).
 __device__ __forceinline__ void r2c_pair(double2 zk, double2 zkp, double2 wk,

@@ -24,7 +24,10 @@ field_kernel(FieldParams p)
     field_body<WAKE, CSR>(p, blockIdx.x, lds);
 }
 
-size_t field_lds_bytes(uint32_t M) { return sizeof(double2) * (M + M / 2); }
+size_t field_lds_bytes(uint32_t M)
+{
+    return sizeof(double2) * (fft_pow2(M) ? M + M / 2 : 2 * M);
+}
 
 int launch_field(const FieldParams& p, bool wake, bool csr, uint32_t blocks,
                  hipStream_t st)
@@ -273,9 +276,9 @@ int vfps_field_create(vfps_mesh* ps, vfps_impedance* z,
     VFPS_CHECK_ARG(nbuckets == ps->nb,
                    "ElectricField: one bucket number per bunch expected");
     const uint32_t nmax = z->n;
-    VFPS_CHECK_ARG(nmax >= 4 && nmax <= 8192 && (nmax & (nmax - 1)) == 0,
-                   "ElectricField: FFT length (impedance size) must be a "
-                   "power of two in [4, 8192]");
+    VFPS_CHECK_ARG(nmax >= 4 && nmax <= 8192 && nmax % 2 == 0,
+                   "ElectricField: FFT length (impedance size) must be even, "
+                   "in [4, 8192]");
     for (uint32_t b = 0; b < nbuckets; b++) {
         VFPS_CHECK_ARG((size_t)buckets[b] * spacing_bins + ps->n <= nmax,
                        "ElectricField: bunch does not fit the padded array");

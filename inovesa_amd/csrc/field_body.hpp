@@ -16,7 +16,8 @@ namespace vfps_impl {
 // The transforms run in FP64 (fft.hpp); the spectrum F, the product
 // W = Z F and the wake potential are rounded to FP32 where the reference
 // holds them as float, and the float arithmetic on them follows the
-// reference.  lds: M + M/2 double2 (data, twiddle half table).  Runs with
+// reference.  lds: field_lds_bytes(M): M + M/2 double2 (data, twiddle half
+// table) for a power-of-two M, else 2 M (data, DFT scratch).  Runs with
 // FFT_THREADS threads.
 constexpr int FIELD_PAIRS = 3;  // (k, M-k) pairs per thread: M <= 4096
 
@@ -35,13 +36,16 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
     __shared__ double red[16];
     VFPS_FSTAMP(0);
     const uint32_t M = p.M;
+    // power-of-two M: radix-4/2 FFT with the half twiddle table in LDS;
+    // other M: direct DFT with M entries of LDS scratch
+    const bool pow2 = fft_pow2(M);
     double2* A = lds;
     double2* twl = lds + M;
     double* Ad = reinterpret_cast<double*>(A);
     for (uint32_t i = threadIdx.x; i < M; i += blockDim.x) {
         A[i] = make_double2(0.0, 0.0);
     }
-    if (WAKE || CSR) {
+    if ((WAKE || CSR) && pow2) {
         for (uint32_t i = threadIdx.x; i < M / 2; i += blockDim.x) {
             twl[i] = p.tw_m[i];
         }
@@ -119,7 +123,11 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
         return;
     }
     VFPS_FSTAMP(1);
-    fft_lds<false>(A, M, twl);
+    if (pow2) {
+        fft_lds<false>(A, M, twl);
+    } else {
+        dft_lds<false>(A, twl, M, p.tw_m);
+    }
     VFPS_FSTAMP(2);
     double2* Zf = A;
     // CSR power: each bin's float term as the reference forms it, summed
@@ -210,7 +218,11 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
     }
     __syncthreads();
     VFPS_FSTAMP(3);
-    fft_lds<true>(A, M, twl);
+    if (pow2) {
+        fft_lds<true>(A, M, twl);
+    } else {
+        dft_lds<true>(A, twl, M, p.tw_m);
+    }
     VFPS_FSTAMP(4);
     const double* yd = reinterpret_cast<const double*>(A);
     for (uint32_t b = 0; b < p.nb; b++) {

@@ -8,6 +8,7 @@
 // accumulated in the reference's order with separate multiply and add
 // roundings, so outputs equal the reference CPU path bit for bit.
 #include <cmath>
+#include <random>
 #include <stdexcept>
 
 #include "common.hpp"
@@ -254,46 +255,96 @@ std::vector<float> drift_offsets(const Ruler& ax0, const Ruler& ax1,
     return off;
 }
 
-// RFKickMap offsets, linear (src/SM/RFKickMap.cpp:59-69)
-std::vector<float> rfkick_offsets_linear(const Ruler& ax0, float angle,
-                                         double qscale, float f_RF)
+// _bl2phase of src/RFKickMap.cpp:30,53: scale("Meter")/c*f_RF*2pi in
+// double, held as float
+static float rf_bl2phase(double qscale, float f_RF)
 {
     const double pi = 3.141592653589793238462643383279502884;
-    const float bl2phase = (float)((double)(float)qscale / 2.99792458e8
-                                   * (double)f_RF * (2.0 * pi));
-    const float phaseoffs = 0.0f - 0.0f;
+    return (float)((double)(float)qscale / 2.99792458e8 * (double)f_RF *
+                   (2.0 * pi));
+}
+
+// RFKickMap::_calcKick(phase, ampl), linear (src/SM/RFKickMap.cpp:59-69)
+std::vector<float> rfkick_offsets_linear_at(const Ruler& ax0, float angle,
+                                            double qscale, float f_RF,
+                                            float syncphase, float phase,
+                                            float ampl)
+{
+    const float bl2phase = rf_bl2phase(qscale, f_RF);
+    const float phaseoffs = syncphase - phase;
     const float xcenter = ax0.zerobin;
     std::vector<float> off(ax0.steps);
     for (uint32_t x = 0; x < ax0.steps; x++) {
         float o = std::tan(angle) * (xcenter - (float)x);
         o = o + std::tan(angle) * phaseoffs / bl2phase / ax0.delta;
-        o = o * 1.0f;
+        o = o * ampl;
         off[x] = o;
     }
     return off;
 }
 
-// RFKickMap offsets, sinusoidal (src/SM/RFKickMap.cpp:70-75)
-std::vector<float> rfkick_offsets(const Ruler& ax0, const Ruler& ax1,
-                                  float revolutionpart, float V_RF,
-                                  float f_RF, float V0, double qscale,
-                                  float pscale)
+// RFKickMap::_calcKick(phase, ampl), sinusoidal (src/SM/RFKickMap.cpp:70-75)
+std::vector<float> rfkick_offsets_at(const Ruler& ax0, const Ruler& ax1,
+                                     float revolutionpart, float V_RF,
+                                     float f_RF, float V0, double qscale,
+                                     float pscale, float phase, float ampl)
 {
-    const double pi = 3.141592653589793238462643383279502884;
-    const float bl2phase = (float)((double)(float)qscale / 2.99792458e8
-                                   * (double)f_RF * (2.0 * pi));
-    const float syncphase = std::asin(V0 / V_RF);
-    const float ampl = 1.0f;
+    const float bl2phase = rf_bl2phase(qscale, f_RF);
     std::vector<float> off(ax0.steps);
     for (uint32_t x = 0; x < ax0.steps; x++) {
         float a = -ampl * V_RF;
-        float s = std::sin(ax0.data[x] * bl2phase + syncphase);
+        float s = std::sin(ax0.data[x] * bl2phase + phase);
         float o = revolutionpart * (a * s + V0);
         o = o / ax1.delta;
         o = o / pscale;
         off[x] = o;
     }
     return off;
+}
+
+// RFKickMap offsets at construction: _calcKick(_syncphase), linear
+// (_syncphase 0) and sinusoidal (_syncphase = asin(V0/V_RF))
+std::vector<float> rfkick_offsets_linear(const Ruler& ax0, float angle,
+                                         double qscale, float f_RF)
+{
+    return rfkick_offsets_linear_at(ax0, angle, qscale, f_RF, 0.0f, 0.0f, 1.0f);
+}
+
+std::vector<float> rfkick_offsets(const Ruler& ax0, const Ruler& ax1,
+                                  float revolutionpart, float V_RF,
+                                  float f_RF, float V0, double qscale,
+                                  float pscale)
+{
+    const float syncphase = std::asin(V0 / V_RF);
+    return rfkick_offsets_at(ax0, ax1, revolutionpart, V_RF, f_RF, V0, qscale,
+                             pscale, syncphase, 1.0f);
+}
+
+// DynamicRFKickMap::__calcModulation (src/SM/DynamicRFKickMap.cpp:80-95):
+// per step, a normally distributed phase and amplitude noise and a
+// sinusoidal phase modulation, drawn in the reference's order from the
+// same generators (std::mt19937, std::normal_distribution<float>), seeded
+// by the caller where the reference takes std::random_device.  [steps][2].
+std::vector<float> dynrf_modulation(uint32_t steps, uint32_t seed,
+                                    float syncphase, double revolutionpart,
+                                    float phasespread, float amplspread,
+                                    float modampl, double modtimeincrement)
+{
+    const double two_pi = 6.283185307179586476925286766559005768;
+    const float phasenoise = (float)(phasespread / std::sqrt(revolutionpart));
+    const float amplnoise = (float)(amplspread / std::sqrt(revolutionpart));
+    const float modtimedelta = (float)(two_pi * modtimeincrement);
+    std::mt19937 prng(seed);
+    std::normal_distribution<float> dist(0.0f, 1.0f);
+    std::vector<float> rv(2 * (size_t)steps);
+    for (uint32_t i = 0; i < steps; i++) {
+        const float pn = dist(prng) * phasenoise;
+        const float an = dist(prng) * amplnoise;
+        const float pm = modampl * std::sin(modtimedelta * (float)i);
+        rv[2 * (size_t)i] = syncphase + pn + pm;
+        rv[2 * (size_t)i + 1] = 1.0f + an;
+    }
+    return rv;
 }
 
 // FokkerPlanckMap weights (src/SM/FokkerPlanckMap.cpp:31-118); the
@@ -657,11 +708,146 @@ int vfps_map_update(vfps_map* map)
     return launch_wake_potential(map->field, map->offset);
 }
 
+// DynamicRFKickMap::apply (src/SM/DynamicRFKickMap.cpp:103-111): the RF
+// offsets of the next queue entry (computed on the host in the reference's
+// float arithmetic, staged through pinned memory), then the kick
+static int dynrf_stage(vfps_map* m)
+{
+    VFPS_CHECK_ARG((size_t)m->mod_next * 2 < m->modulation.size(),
+                   "DynamicRFKickMap::apply: more steps than the modulation "
+                   "queue holds");
+    const float phase = m->modulation[2 * (size_t)m->mod_next];
+    const float ampl = m->modulation[2 * (size_t)m->mod_next + 1];
+    const vfps_mesh* ps = m->in;
+    const std::vector<float> off =
+        m->rf_linear
+            ? rfkick_offsets_linear_at(ps->ax0, m->rf_angle, ps->qscale,
+                                       m->rf_f, m->rf_syncphase, phase, ampl)
+            : rfkick_offsets_at(ps->ax0, ps->ax1, m->rf_revolutionpart,
+                                m->rf_V, m->rf_f, m->rf_V0, ps->qscale,
+                                (float)ps->pscale, phase, ampl);
+    hipStream_t st = ps->dev->stream;
+    // the previous upload from the staging buffer has finished reading it
+    VFPS_HIP(hipEventSynchronize(m->stage_done));
+    std::memcpy(m->stage_h, off.data(), sizeof(float) * off.size());
+    VFPS_HIP(hipMemcpyAsync(m->offset, m->stage_h, sizeof(float) * off.size(),
+                            hipMemcpyHostToDevice, st));
+    VFPS_HIP(hipEventRecord(m->stage_done, st));
+    m->mod_next++;
+    return VFPS_OK;
+}
+
 int vfps_map_apply(vfps_map* map)
 {
     VFPS_CHECK_ARG(map, "apply: NULL map");
+    if (map->dynrf) {
+        VFPS_TRY_BEGIN
+        int rc = dynrf_stage(map);
+        if (rc != VFPS_OK) return rc;
+        VFPS_TRY_END
+    }
     return launch_map(map, map->in->dev->stream, map->in->data,
                       map->out->data);
+}
+
+static int dynrf_create(vfps_mesh* in, vfps_mesh* out, bool linear,
+                        float angle, double revolutionpart, float V_RF,
+                        float f_RF, float V0, float phasespread,
+                        float amplspread, float modampl,
+                        double modtimeincrement, uint32_t steps, uint32_t seed,
+                        int interp, int clamp, vfps_map** map)
+{
+    VFPS_CHECK_ARG(in && out && map && steps > 0 && revolutionpart > 0,
+                   "DynamicRFKickMap: bad argument");
+    VFPS_CHECK_ARG(linear || V_RF != 0, "DynamicRFKickMap: V_RF is zero");
+    int rc = vfps_map_create_kick(in, out, interp, clamp, 1, map);
+    if (rc != VFPS_OK) return rc;
+    vfps_map* m = *map;
+    VFPS_TRY_BEGIN
+    m->dynrf = true;
+    m->rf_linear = linear;
+    m->rf_angle = angle;
+    m->rf_revolutionpart = (float)revolutionpart;
+    m->rf_V = V_RF;
+    m->rf_f = f_RF;
+    m->rf_V0 = V0;
+    m->rf_syncphase = linear ? 0.0f : std::asin(V0 / V_RF);
+    m->modulation = dynrf_modulation(steps, seed, m->rf_syncphase,
+                                     revolutionpart, phasespread,
+                                     amplspread, modampl, modtimeincrement);
+    if (hipHostMalloc(&m->stage_h, sizeof(float) * in->n,
+                      hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&m->stage_done, hipEventDisableTiming) !=
+            hipSuccess) {
+        vfps_map_destroy(m);
+        *map = nullptr;
+        return fail(VFPS_ERR_NOMEM, "DynamicRFKickMap: staging buffer");
+    }
+    VFPS_HIP(hipEventRecord(m->stage_done, in->dev->stream));
+    // the static RF kick of the constructor (RFKickMap::_calcKick(_syncphase))
+    const auto off =
+        linear ? rfkick_offsets_linear(in->ax0, angle, in->qscale, f_RF)
+               : rfkick_offsets(in->ax0, in->ax1, (float)revolutionpart, V_RF,
+                                f_RF, V0, in->qscale, (float)in->pscale);
+    return upload_offset(m, off);
+    VFPS_TRY_END
+}
+
+int vfps_map_create_dynrfkick(vfps_mesh* in, vfps_mesh* out,
+                              double revolutionpart, float V_RF, float f_RF,
+                              float V0, float phasespread, float amplspread,
+                              float modampl, double modtimeincrement,
+                              uint32_t steps, uint32_t seed, int interp,
+                              int clamp, vfps_map** map)
+{
+    return dynrf_create(in, out, false, 0.0f, revolutionpart, V_RF, f_RF, V0,
+                        phasespread, amplspread, modampl, modtimeincrement,
+                        steps, seed, interp, clamp, map);
+}
+
+int vfps_map_create_dynrfkick_linear(vfps_mesh* in, vfps_mesh* out,
+                                     float angle, double revolutionpart,
+                                     float f_RF, float phasespread,
+                                     float amplspread, float modampl,
+                                     double modtimeincrement, uint32_t steps,
+                                     uint32_t seed, int interp, int clamp,
+                                     vfps_map** map)
+{
+    return dynrf_create(in, out, true, angle, revolutionpart, 0.0f, f_RF, 0.0f,
+                        phasespread, amplspread, modampl, modtimeincrement,
+                        steps, seed, interp, clamp, map);
+}
+
+int vfps_map_get_modulation(const vfps_map* map, uint32_t first,
+                            uint32_t count, float* out, uint32_t* applied)
+{
+    VFPS_CHECK_ARG(map && map->dynrf, "getPastModulation: not a "
+                                      "DynamicRFKickMap");
+    VFPS_CHECK_ARG(!count || out, "getPastModulation: NULL output");
+    VFPS_CHECK_ARG(((size_t)first + count) * 2 <= map->modulation.size(),
+                   "getPastModulation: beyond the queue");
+    if (count) {
+        std::memcpy(out, map->modulation.data() + 2 * (size_t)first,
+                    sizeof(float) * 2 * count);
+    }
+    if (applied) *applied = map->mod_next;
+    return VFPS_OK;
+}
+
+int vfps_dynrf_modulation(uint32_t steps, uint32_t seed, float syncphase,
+                          double revolutionpart, float phasespread,
+                          float amplspread, float modampl,
+                          double modtimeincrement, float* out)
+{
+    VFPS_CHECK_ARG(out && revolutionpart > 0, "vfps_dynrf_modulation: bad "
+                                              "argument");
+    VFPS_TRY_BEGIN
+    const auto q = dynrf_modulation(steps, seed, syncphase, revolutionpart,
+                                    phasespread, amplspread, modampl,
+                                    modtimeincrement);
+    std::memcpy(out, q.data(), sizeof(float) * q.size());
+    return VFPS_OK;
+    VFPS_TRY_END
 }
 
 int vfps_map_destroy(vfps_map* map)
@@ -671,6 +857,11 @@ int vfps_map_destroy(vfps_map* map)
     hipFree(map->offset);
     hipFree(map->fp_w);
     hipFree(map->fp_i);
+    if (map->stage_done) {
+        hipEventSynchronize(map->stage_done);
+        hipEventDestroy(map->stage_done);
+    }
+    if (map->stage_h) hipHostFree(map->stage_h);
     delete map;
     return VFPS_OK;
 }

@@ -410,6 +410,7 @@ static void step_free(vfps_step* s)
     hipFree(s->filling);
     hipFree(s->counter);
     hipFree(s->err);
+    if (s->err_h) hipHostFree(s->err_h);
     hipFree(s->csr_hist);
     hipFree(s->prof_hist);
     hipFree(s->wake_g);
@@ -766,9 +767,9 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
         s->M = s->wfield->nmax / 2;
     } else if (desc->use_csr) {
         VFPS_CHECK_ARG(zcsr->n >= n && zcsr->n >= 4 && zcsr->n <= 8192 &&
-                           (zcsr->n & (zcsr->n - 1)) == 0,
-                       "vfps_step_create: CSR impedance length must be a "
-                       "power of two >= the mesh size");
+                           zcsr->n % 2 == 0,
+                       "vfps_step_create: CSR impedance length must be even, "
+                       "at least the mesh size and at most 8192");
         s->M = zcsr->n / 2;
     } else {
         uint32_t m = 2;
@@ -820,6 +821,8 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
     VFPS_HIP(hipMalloc(&s->filling, sizeof(double) * 2));
     VFPS_HIP(hipMalloc(&s->counter, sizeof(uint32_t) * 2));
     VFPS_HIP(hipMalloc(&s->err, sizeof(uint32_t)));
+    VFPS_HIP(hipHostMalloc(&s->err_h, sizeof(uint32_t), hipHostMallocDefault));
+    *s->err_h = 0;
     VFPS_HIP(hipMalloc(&s->csr_power, sizeof(float)));
     VFPS_HIP(hipMemset(s->filling, 0, sizeof(double) * 2));
     VFPS_HIP(hipMemset(s->counter, 0, sizeof(uint32_t) * 2));
@@ -930,9 +933,11 @@ int vfps_step_run(vfps_step* s, uint32_t nsteps)
         if ((rc = launch_step_finalize(s->inst_d, 1, s->parity, st)) != VFPS_OK)
             return rc;
     }
+    // the error flag comes back in the same stream: one wait for both
+    VFPS_HIP(hipMemcpyAsync(s->err_h, s->err, sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, st));
     VFPS_HIP(hipStreamSynchronize(st));
-    uint32_t err = 0;
-    VFPS_HIP(hipMemcpy(&err, s->err, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t err = *s->err_h;
     if (err) {
         return fail(VFPS_ERR_INVALID, "fused step: a tap fell outside the "
                                       "halo rows (internal error)");

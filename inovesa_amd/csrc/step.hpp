@@ -122,6 +122,7 @@ struct vfps_step {
     double* filling = nullptr;  // [2]
     uint32_t* counter = nullptr;    // [2] (ping-pong in the fused step)
     uint32_t* err = nullptr;
+    uint32_t* err_h = nullptr;  // pinned host copy, read after each run
     float* csr_hist = nullptr;
     float* prof_hist = nullptr;
     uint32_t hist_cap = 0;

@@ -20,7 +20,8 @@ __all__ = ["Device", "PhaseSpace", "KickMap", "DriftMap", "RFKickMap",
            "ElectricField", "Step", "derive", "airy", "VfpsError",
            "VfpsInvalid", "InterpolationType", "Axis", "FPType",
            "DerivationType", "toeplitz_kernels", "Batch", "MLP",
-           "gemm_bf16", "round_bf16", "wake_potential_noalloc"]
+           "gemm_bf16", "round_bf16", "wake_potential_noalloc",
+           "DynamicRFKickMap", "dynrf_modulation"]
 
 
 class InterpolationType:  # inc/SM/SourceMap.hpp:22-27
@@ -267,6 +268,58 @@ class RFKickMap(KickMap):
                  V_RF, f_RF, V0, it, int(clamp), C.byref(h))
         SourceMap.__init__(self, h, src, dst)
         self.n_off = src.n * src.nb
+
+
+class DynamicRFKickMap(KickMap):
+    """src/SM/DynamicRFKickMap.cpp: an RF kick recomputed before every
+    apply() from a precomputed queue of (phase, amplitude) modulations.
+    Sinusoidal RF (revolutionpart, V_RF, f_RF, V0) unless `angle` is given
+    (linear RF).  seed: the noise generator's seed (the reference draws it
+    from std::random_device)."""
+
+    def __init__(self, src, dst, revolutionpart, f_RF, phasespread,
+                 amplspread, modampl, modtimeincrement, steps, seed=0,
+                 V_RF=None, V0=None, angle=None, it=4, clamp=False):
+        h = C.c_void_p()
+        if angle is not None:
+            call("vfps_map_create_dynrfkick_linear", src.h, dst.h, angle,
+                 revolutionpart, f_RF, phasespread, amplspread, modampl,
+                 modtimeincrement, steps, seed, it, int(clamp), C.byref(h))
+        else:
+            call("vfps_map_create_dynrfkick", src.h, dst.h, revolutionpart,
+                 V_RF, f_RF, V0, phasespread, amplspread, modampl,
+                 modtimeincrement, steps, seed, it, int(clamp), C.byref(h))
+        SourceMap.__init__(self, h, src, dst)
+        self.n_off = src.n * src.nb
+        self.steps = steps
+        self._read = 0
+
+    def modulation(self, first=0, count=None):
+        """Queue entries [first, first+count) as [count][2] (phase,
+        amplitude), and the number applied so far."""
+        count = self.steps - first if count is None else count
+        out = np.empty((count, 2), np.float32)
+        applied = C.c_uint32()
+        call("vfps_map_get_modulation", self.h, first, count, _p(out),
+             C.byref(applied))
+        return out, applied.value
+
+    def getPastModulation(self):
+        """Modulations applied since the last call
+        (src/SM/DynamicRFKickMap.cpp:113-119)."""
+        _, applied = self.modulation(0, 0)
+        out, _ = self.modulation(self._read, applied - self._read)
+        self._read = applied
+        return out
+
+
+def dynrf_modulation(steps, seed, syncphase, revolutionpart, phasespread,
+                     amplspread, modampl, modtimeincrement):
+    """The modulation queue of DynamicRFKickMap alone (host only)."""
+    out = np.empty((steps, 2), np.float32)
+    call("vfps_dynrf_modulation", steps, seed, syncphase, revolutionpart,
+         phasespread, amplspread, modampl, modtimeincrement, _p(out))
+    return out
 
 
 class RotationMap(SourceMap):

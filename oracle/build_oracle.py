@@ -3,6 +3,8 @@
 Compiles oracle/vfps_oracle.c twice with gcc:
   oracle/liboracle_f32.so  -- real = float  (parity target)
   oracle/liboracle_f64.so  -- real = double (FP64 drift oracle)
+and oracle/dynrf_oracle.cpp with g++:
+  oracle/liboracle_dynrf.so -- the DynamicRFKickMap modulation queue
 
 -ffp-contract=off keeps every rounding where the reference source puts it.
 -march=x86-64-v3 (not -march=native) so the library also runs on the host
@@ -34,6 +36,17 @@ def build(verbose=False):
         subprocess.run(cmd, check=True)
         os.replace(tmp, out)
         outs.append(out)
+    # the DynamicRFKickMap modulation queue (C++: the reference's generators
+    # are the standard library's)
+    out = os.path.join(HERE, "liboracle_dynrf.so")
+    tmp = out + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++14", "-fPIC", "-shared", "-ffp-contract=off",
+           "-Wall", os.path.join(HERE, "dynrf_oracle.cpp"), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, out)
+    outs.append(out)
     return outs
 
 

@@ -28,6 +28,25 @@ class SimCfg(C.Structure):
     pass
 
 
+_DYNRF = None
+
+
+def _dynrf_lib():
+    global _DYNRF
+    if _DYNRF is None:
+        path = os.path.join(HERE, "liboracle_dynrf.so")
+        if not os.path.exists(path):
+            from oracle.build_oracle import build
+            build()
+        L = C.CDLL(path)
+        L.or_dynrf_modulation.argtypes = [_u32, _u32, C.c_float, _f64,
+                                          C.c_float, C.c_float, C.c_float,
+                                          _f64, _vp]
+        L.or_mt19937_10000th.restype = _u32
+        _DYNRF = L
+    return _DYNRF
+
+
 class Oracle:
     def __init__(self, fp64=False, path=None):
         """path: another build of the same source (the natively compiled
@@ -92,6 +111,10 @@ class Oracle:
         L.or_drift_offsets.argtypes = [_u32, _vp, R, _vp, _u32, R, R, _vp]
         L.or_rfkick_offsets_linear.argtypes = [_u32, R, R, R, R, R, _vp]
         L.or_rfkick_offsets.argtypes = [_u32, _vp, R, R, R, R, R, R, R, _vp]
+        L.or_rfkick_offsets_at.argtypes = [_u32, _vp, R, R, R, R, R, R, R,
+                                           R, R, _vp]
+        L.or_rfkick_offsets_linear_at.argtypes = [_u32, R, R, R, R, R, R, R,
+                                                  R, _vp]
         L.or_rotation_hinfo.argtypes = [_u32, _u32, _vp, R, R, _vp, R, R, R,
                                         R, _u32, _u32, _u32, _vp]
         L.or_rotation_build.restype = C.c_int
@@ -264,6 +287,40 @@ class Oracle:
                                    V_RF, f_RF, V0, qscale, delta1, pscale,
                                    _ptr(out))
         return out
+
+    def rfkick_offsets_at(self, axis0, revolutionpart, V_RF, f_RF, V0,
+                          qscale, delta1, pscale, phase, ampl):
+        """RFKickMap::_calcKick(phase, ampl), sinusoidal."""
+        axis0 = self.arr(axis0)
+        out = np.empty_like(axis0)
+        self.lib.or_rfkick_offsets_at(len(axis0), _ptr(axis0), revolutionpart,
+                                      V_RF, f_RF, V0, qscale, delta1, pscale,
+                                      phase, ampl, _ptr(out))
+        return out
+
+    def rfkick_offsets_linear_at(self, n, zerobin0, angle, qscale, f_RF,
+                                 delta0, syncphase, phase, ampl):
+        """RFKickMap::_calcKick(phase, ampl), linear."""
+        out = np.empty(n, self.real)
+        self.lib.or_rfkick_offsets_linear_at(n, zerobin0, angle, qscale, f_RF,
+                                             delta0, syncphase, phase, ampl,
+                                             _ptr(out))
+        return out
+
+    @staticmethod
+    def dynrf_modulation(steps, seed, syncphase, revolutionpart, phasespread,
+                         amplspread, modampl, modtimeincrement):
+        """DynamicRFKickMap's modulation queue [steps][2] (phase, ampl)."""
+        L = _dynrf_lib()
+        out = np.empty((steps, 2), np.float32)
+        L.or_dynrf_modulation(steps, seed, syncphase, revolutionpart,
+                              phasespread, amplspread, modampl,
+                              modtimeincrement, _ptr(out))
+        return out
+
+    @staticmethod
+    def mt19937_10000th():
+        return _dynrf_lib().or_mt19937_10000th()
 
     def rotation(self, data, axis0, d0, zb0, axis1, d1, zb1, angle, it,
                  clamp, rotmapsize):

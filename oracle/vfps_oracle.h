@@ -123,6 +123,15 @@ void or_rfkick_offsets_linear(uint32_t n, real zerobin0, real angle,
 void or_rfkick_offsets(uint32_t n, const real* axis0, real revolutionpart,
                        real V_RF, real f_RF, real V0, real qscale,
                        real delta1, real pscale, real* offset);
+/* _calcKick(phase, ampl) of src/SM/RFKickMap.cpp:59-75 (DynamicRFKickMap) */
+void or_rfkick_offsets_at(uint32_t n, const real* axis0, real revolutionpart,
+                          real V_RF, real f_RF, real V0, real qscale,
+                          real delta1, real pscale, real phase, real ampl,
+                          real* offset);
+void or_rfkick_offsets_linear_at(uint32_t n, real zerobin0, real angle,
+                                 real qscale, real f_RF, real delta0,
+                                 real syncphase, real phase, real ampl,
+                                 real* offset);
 
 /* ---- src/SM/RotationMap.cpp:190-262 (genHInfo) ------------------------ */
 void or_rotation_hinfo(uint32_t xsize, uint32_t ysize,

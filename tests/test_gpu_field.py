@@ -106,13 +106,6 @@ def test_padding_and_constant_wake(V, orc):
     assert np.abs(wp - prof * f.getWakeScaling()).max() < 1e-6
 
 
-def test_nonpower_of_two_fft_rejected(V):
-    ps = V.PhaseSpace(32, -12, 12, 3, -12, 12, 4)
-    z = V.ConstImpedance(768, 1e9, 1.0)
-    with pytest.raises(V.VfpsInvalid):
-        V.ElectricField(ps, z, [0], 256, 1e6, 0.1, 1.0)
-
-
 def test_forward_wake_causal(V):
     # test/TestElectricField.cpp:175-239
     n, ps, z, f = _field_fixture(V, [1.0], 0.0)
@@ -180,3 +173,105 @@ def test_csr_vs_oracle(V, orc, n, fc):
     exact = terms.sum()
     assert abs(power[0] - exact) <= 1e-6 * abs(exact)
     assert abs(rpower[0] - exact) <= nmax * 2.0 ** -23 * np.abs(terms).sum()
+
+
+def _multibunch_fixture(V, pattern, zval):
+    """test/TestElectricField.cpp:22-69: n = 32, 8 bins of bunch spacing
+    per mesh width (spacing 256 bins), one bucket per pattern entry (the
+    FFT length is 256 per bucket: 768 for three), filled buckets numbered
+    from the end of the pattern."""
+    n, spacing = 32, 256
+    spaced = spacing * len(pattern)
+    buckets = [len(pattern) - 1 - i for i, f in enumerate(pattern) if f > 0]
+    bunches = [f for f in pattern if f > 0]
+    ps = V.PhaseSpace(n, -12, 12, 3, -12, 12, 4, filling=bunches)
+    z = V.ConstImpedance(spaced, 1e9, zval)
+    ws = V.ElectricField.scaling(1e-3, 1e-3, 3.0, ps.getDelta(1), 1e3, 1e9)
+    f = V.ElectricField(ps, z, buckets, spacing, 1e6, 0.1, ws)
+    return n, spacing, buckets, ps, z, f
+
+
+PATTERNS = [[1.0], [0.75, 0.0, 0.25]]
+
+
+@pytest.mark.parametrize("pattern", PATTERNS)
+def test_multibunch_padding(V, orc, pattern):
+    # test/TestElectricField.cpp:110-141
+    n, spacing, buckets, ps, z, f = _multibunch_fixture(V, pattern, 1.0)
+    f.padBunchProfiles()
+    pad = f.getPaddedBunchProfiles()
+    g = orc.gaus(ps.getAxis(0)).astype(np.float64)
+    exp = np.zeros(spacing * len(pattern))
+    for bucket, norm in enumerate(reversed(pattern)):
+        exp[bucket * spacing: bucket * spacing + n] = norm * g
+    np.testing.assert_allclose(pad, exp, rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("pattern", PATTERNS)
+def test_multibunch_constant_wake(V, orc, pattern):
+    """test/TestElectricField.cpp:147-168: with a constant impedance of 1
+    the wake potential of every bunch is its profile times the scaling
+    (within 1e-6), here with an FFT length of 768 (3 x 256) for the
+    three-bucket pattern; and it equals the oracle's."""
+    n, spacing, buckets, ps, z, f = _multibunch_fixture(V, pattern, 1.0)
+    wp = f.wakePotential()
+    prof = ps.getProjection(0)
+    assert wp.shape == (len(buckets), n)
+    assert np.abs(wp - prof * f.getWakeScaling()).max() < 1e-6
+    ws = V.ElectricField.scaling(1e-3, 1e-3, 3.0, ps.getDelta(1), 1e3, 1e9)
+    ref, pad = orc.wake_potential(prof, z.data(), buckets, spacing,
+                                  np.float32(ws))
+    assert rel_linf(wp, ref) <= 1e-5
+    assert rel_linf(f.getPaddedBunchProfiles(), pad) == 0
+
+
+@pytest.mark.parametrize("pattern", PATTERNS)
+def test_multibunch_forward_wake_causal(V, pattern):
+    """test/TestElectricField.cpp:175-239 on every bunch: free-space CSR
+    over 256 or 768 bins; a change of the profile from x = n/2 on moves the
+    wake only from there on."""
+    n, spacing, buckets, ps, z, f = _multibunch_fixture(V, pattern, 0.0)
+    z += V.FreeSpaceCSR(spacing * len(pattern), 1e6, 1e12)
+    w0 = f.wakePotential()
+    prof = ps.getProjection(0)
+    c = n // 2
+    for b in range(len(buckets)):
+        dl = np.float32(0.1) * prof[b, c]
+        prof[b, c] -= dl
+        prof[b, c + 1] += dl
+        prof[b, c + 2] += dl
+        prof[b, c + 3] -= dl
+    ps.setProjection(0, prof)
+    w1 = f.wakePotential()
+    for b in range(len(buckets)):
+        assert np.abs(w0[b] - w1[b]).max() > 2e-8
+        assert np.abs(w0[b, :c] - w1[b, :c]).max() < 2e-8
+
+
+@pytest.mark.parametrize("nmax", [768, 1536, 3072])
+def test_nonpower_of_two_wake_and_csr_vs_oracle(V, orc, nmax):
+    """FFT lengths that are not powers of two (RoundPadding off: 3 x 2^k)
+    through the direct DFT: wake potential and CSR spectrum of two bunches
+    against the oracle."""
+    n = 128 if nmax < 3072 else 384
+    d = V.derive(grid_size=n, vacuum_gap=0.032)
+    f0, fmax = np.float32(d["f0"]), np.float32(d["fmax"])
+    z = V.ParallelPlatesCSR(nmax, f0, fmax, 0.032)
+    z += V.ConstImpedance(nmax, fmax, 50 + 10j)
+    data = np.concatenate([noisy_mesh(n, 41), noisy_mesh(n, 42)])
+    ps = V.PhaseSpace(n, d["qmin"], d["qmax"], d["qscale"], d["pmin"],
+                      d["pmax"], d["pscale"], filling=[0.6, 0.4], data=data)
+    spacing = (nmax - n) // 2
+    buckets = [2, 0]
+    f = V.ElectricField(ps, z, buckets, spacing, 9e6, d["revolutionpart"],
+                        d["wakescaling"])
+    wp = f.wakePotential()
+    prof = ps.getProjection(0)
+    ref, pad = orc.wake_potential(prof, z.data(), buckets, spacing,
+                                  np.float32(d["wakescaling"]))
+    assert rel_linf(wp, ref) <= 1e-5
+    power, spec = f.updateCSR(np.float32(23e9))
+    rspec, rpower = orc.update_csr(prof, z.data(), ps.getDelta(0),
+                                   np.float32(d["qscale"]), np.float32(23e9))
+    assert rel_linf(spec, rspec) <= 1e-5
+    np.testing.assert_allclose(power, rpower, rtol=2e-5)

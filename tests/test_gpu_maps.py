@@ -227,3 +227,48 @@ def test_projection_integral_normalize(V, orc):
     ps.normalize()
     ps.updateXProjection()
     assert abs(ps.integrate()[1] - 1) < 1e-6
+
+
+@pytest.mark.parametrize("linear", [False, True])
+def test_dynamic_rf_kick_bitexact(V, orc, linear):
+    """DynamicRFKickMap (src/SM/DynamicRFKickMap.cpp:97-111): before every
+    apply the RF kick is recomputed from the next (phase, amplitude) of the
+    queue; each step's offsets and kicked mesh equal the oracle's bit for
+    bit, getPastModulation returns the applied entries once, and an apply
+    past the queue fails."""
+    n, steps, seed = 256, 6, 42
+    d = noisy_mesh(n, 21)
+    qscale, pscale = 2.0153e-3, 6.11e5
+    a = V.PhaseSpace(n, -6.0, 6.0, qscale, -6.0, 6.0, pscale, data=d)
+    b = V.PhaseSpace(n, -6.0, 6.0, qscale, -6.0, 6.0, pscale)
+    ax0, d0, zb0 = orc.ruler(n, -6.0, 6.0)
+    ax1, d1, _ = orc.ruler(n, -6.0, 6.0)
+    revpart, f_RF, Vrf, V0 = 0.0494, 4.5e8, 998.86e3, 47.65e3
+    noise = dict(phasespread=2e-2, amplspread=1e-2, modampl=5e-2,
+                 modtimeincrement=1.0 / 17.0)
+    ang = np.float32(2 * math.pi / 1000)
+    if linear:
+        m = V.DynamicRFKickMap(a, b, revpart, f_RF, steps=steps, seed=seed,
+                               angle=ang, **noise)
+        sync = np.float32(0.0)
+    else:
+        m = V.DynamicRFKickMap(a, b, revpart, f_RF, steps=steps, seed=seed,
+                               V_RF=Vrf, V0=V0, **noise)
+        sync = np.float32(math.asin(np.float32(V0) / np.float32(Vrf)))
+    q = orc.dynrf_modulation(steps, seed, sync, revpart, **noise)
+    for k in range(steps):
+        m.apply()
+        if linear:
+            off = orc.rfkick_offsets_linear_at(n, zb0, ang, np.float32(qscale),
+                                               f_RF, d0, sync, q[k, 0],
+                                               q[k, 1])
+        else:
+            off = orc.rfkick_offsets_at(ax0, revpart, Vrf, f_RF, V0,
+                                        np.float32(qscale), d1,
+                                        np.float32(pscale), q[k, 0], q[k, 1])
+        assert np.array_equal(m.getForce()[:n], off), k
+        assert np.array_equal(b.getData(), orc.kick(1, d, off, 4)), k
+    assert np.array_equal(m.getPastModulation(), q)
+    assert m.getPastModulation().shape == (0, 2)
+    with pytest.raises(V.VfpsInvalid):
+        m.apply()
