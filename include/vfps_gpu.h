@@ -269,6 +269,10 @@ VFPS_API int vfps_field_destroy(vfps_field* f);
 VFPS_API int vfps_field_pad(vfps_field* f);
 /* getPaddedBunchProfiles: nmax floats; synchronises */
 VFPS_API int vfps_field_get_padded(vfps_field* f, float* out);
+/* getPaddedWakePotential (inc/PS/ElectricField.hpp:170-171): the
+ * unscaled inverse transform of the last wakePotential(), nmax floats;
+ * synchronises */
+VFPS_API int vfps_field_get_padded_wake(vfps_field* f, float* out);
 /* wakePotential (src/PS/ElectricField.cpp:186-253); wp ([nb][n]) may be
  * NULL (the result stays on the device); non-NULL synchronises */
 VFPS_API int vfps_field_wake_potential(vfps_field* f, float* wp);
@@ -385,6 +389,11 @@ typedef struct {
     uint32_t threads;         /* fused step workgroup size (0: default) */
     int32_t no_graph;         /* 1: vfps_step_run launches every step itself
                                  instead of replaying a captured hipGraph */
+    uint32_t renormalize;     /* RenormalizeCharge (src/main.cpp:953-955):
+                                 > 0 renormalises the charge to the set
+                                 bunch population before the maps of every
+                                 step whose count is a multiple of it (the
+                                 fused direct form only) */
 } vfps_step_desc;
 
 /* The FP64 Toeplitz kernels of the direct wake and CSR forms (host only,
@@ -424,6 +433,15 @@ VFPS_API int vfps_step_get_history(vfps_step* s, uint32_t* count,
 VFPS_API int vfps_step_get_mesh(vfps_step* s, float* mesh, float* proj,
                                 float* wake);
 VFPS_API int vfps_step_set_mesh(vfps_step* s, const float* mesh);
+/* The outstep moments of the current state (src/main.cpp:961-967:
+ * integrate(), variance(0), updateYProjection(), variance(1)), as
+ * HDF5File::append reads them (src/IO/HDF5File.cpp:402-418): x projection
+ * and y projection (n each), bunch population (the FP64 integral),
+ * position getMoment(0,0), length getBunchLength(), energy average
+ * getMoment(1,0), spread getEnergySpread(); any may be NULL; synchronises */
+VFPS_API int vfps_step_moments(vfps_step* s, float* xproj, float* yproj,
+                               float* population, float* position,
+                               float* length, float* average, float* spread);
 /* Advance nsteps and time the kernels with HIP events on the step's stream
  * (milliseconds per step): the direct form launches its one kernel nsteps
  * times back to back between two events (ms_maps; ms_wake = 0); the FFT

@@ -78,7 +78,8 @@ class StepDesc(C.Structure):
                 ("spacing_bins", _u32), ("wakescaling", _f32),
                 ("use_csr", _i32), ("fc", _f32), ("fused", _i32),
                 ("tile_rows", _u32), ("fast_projection", _i32),
-                ("wake_method", _u32), ("threads", _u32), ("no_graph", _i32)]
+                ("wake_method", _u32), ("threads", _u32), ("no_graph", _i32),
+                ("renormalize", _u32)]
 
 
 _SIGS = {
@@ -151,6 +152,7 @@ _SIGS = {
     "vfps_airy": (_int, [_vp, _vp, _u32, _vp, _vp]),
     "vfps_wake_potential_noalloc": (_int, [_vp, _vp, _u32, _u32, _vp, _u32,
                                            _vp, _u32, C.c_float, _vp]),
+    "vfps_field_get_padded_wake": (_int, [_vp, _vp]),
     "vfps_field_create": (_int, [_vp, _vp, _vp, _u32, _u32, _f64, _f32, _f32,
                                  _vp]),
     "vfps_field_wakescaling": (_f32, [_f64, _f64, _f64, _f32, _f64, _f64]),
@@ -170,6 +172,7 @@ _SIGS = {
     "vfps_step_get_history": (_int, [_vp, _vp, _vp, _vp]),
     "vfps_step_get_mesh": (_int, [_vp, _vp, _vp, _vp]),
     "vfps_step_set_mesh": (_int, [_vp, _vp]),
+    "vfps_step_moments": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "vfps_step_timing": (_int, [_vp, _u32, _vp, _vp, _vp]),
     "vfps_step_phases": (_int, [_vp, _vp, _u32]),
     "vfps_step_config": (_int, [_vp, _vp, _u32]),

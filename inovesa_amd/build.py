@@ -71,5 +71,34 @@ def build(verbose=False, jobs=None):
     return lib
 
 
+HDF5_PREFIX = os.environ.get("VFPS_HDF5_PREFIX", "/opt/conda")
+
+
+def build_h5(verbose=False):
+    """libvfps_h5.so: the HDF5 result/checkpoint files (host only, g++ over
+    the HDF5 C library of HDF5_PREFIX).  Returns the path, or None when the
+    HDF5 headers or library are absent."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    libdir = os.path.join(HDF5_PREFIX, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and
+            os.path.exists(os.path.join(libdir, "libhdf5.so"))):
+        print(f"libvfps_h5.so not built: no HDF5 under {HDF5_PREFIX}",
+              file=sys.stderr)
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    lib = os.path.join(LIBDIR, "libvfps_h5.so")
+    tmp = lib + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+           "-fvisibility=hidden", "-I", os.path.join(ROOT, "include"),
+           "-I", inc, os.path.join(HERE, "csrc_h5", "h5file.cpp"),
+           "-L", libdir, "-lhdf5", f"-Wl,-rpath,{libdir}", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, lib)
+    return lib
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_h5(verbose="-v" in sys.argv))

@@ -142,6 +142,7 @@ struct vfps_field {
     double f_rev = 0;
     float revolutionpart = 1;
     float* wake = nullptr;       // [nb][n]
+    float* wake_padded = nullptr;  // [nmax] unscaled C2R output
     float* padded = nullptr;     // [nmax]
     double2* tw_m = nullptr;     // e^{-2 pi i t/M}, t < M (M = nmax/2)
     double2* tw_n = nullptr;     // e^{-2 pi i k/nmax}, k < M

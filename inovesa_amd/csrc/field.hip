@@ -127,6 +127,7 @@ int launch_wake_potential(vfps_field* f, float* dst)
     FieldParams p = field_params(f);
     p.wake2 = dst;
     p.padded = f->padded;
+    p.wake_padded = f->wake_padded;
     return launch_field(p, true, false, 1, f->dev->stream);
 }
 
@@ -311,6 +312,7 @@ int vfps_field_create(vfps_mesh* ps, vfps_impedance* z,
     e = e ? e : hipMalloc(&f->buckets_d, sizeof(uint32_t) * nb);
     e = e ? e : hipMalloc(&f->wake, sizeof(float) * nb * n);
     e = e ? e : hipMalloc(&f->padded, sizeof(float) * nmax);
+    e = e ? e : hipMalloc(&f->wake_padded, sizeof(float) * nmax);
     e = e ? e : hipMalloc(&f->tw_m, sizeof(double2) * M);
     e = e ? e : hipMalloc(&f->tw_n, sizeof(double2) * M);
     e = e ? e : hipMalloc(&f->csr_renorm, sizeof(float) * nmax);
@@ -325,6 +327,7 @@ int vfps_field_create(vfps_mesh* ps, vfps_impedance* z,
                    hipMemcpyHostToDevice, st);
     hipMemsetAsync(f->wake, 0, sizeof(float) * nb * n, st);
     hipMemsetAsync(f->padded, 0, sizeof(float) * nmax, st);
+    hipMemsetAsync(f->wake_padded, 0, sizeof(float) * nmax, st);
     hipMemcpyAsync(f->tw_m, twm.data(), sizeof(double2) * M,
                    hipMemcpyHostToDevice, st);
     hipMemcpyAsync(f->tw_n, twn.data(), sizeof(double2) * M,
@@ -350,6 +353,7 @@ int vfps_field_destroy(vfps_field* f)
     hipFree(f->buckets_d);
     hipFree(f->wake);
     hipFree(f->padded);
+    hipFree(f->wake_padded);
     hipFree(f->tw_m);
     hipFree(f->tw_n);
     hipFree(f->csr_renorm);
@@ -388,6 +392,15 @@ int vfps_field_get_padded(vfps_field* f, float* out)
 {
     VFPS_CHECK_ARG(f && out, "getPaddedBunchProfiles: NULL argument");
     VFPS_HIP(hipMemcpyAsync(out, f->padded, sizeof(float) * f->nmax,
+                            hipMemcpyDeviceToHost, f->dev->stream));
+    VFPS_HIP(hipStreamSynchronize(f->dev->stream));
+    return VFPS_OK;
+}
+
+int vfps_field_get_padded_wake(vfps_field* f, float* out)
+{
+    VFPS_CHECK_ARG(f && out, "getPaddedWakePotential: NULL argument");
+    VFPS_HIP(hipMemcpyAsync(out, f->wake_padded, sizeof(float) * f->nmax,
                             hipMemcpyDeviceToHost, f->dev->stream));
     VFPS_HIP(hipStreamSynchronize(f->dev->stream));
     return VFPS_OK;

@@ -20,6 +20,8 @@ struct FieldParams {
     float* wake;              // [nb][n]
     float* wake2;             // optional copy (a kick map's offsets)
     float* padded;            // optional [2M] padded profile
+    float* wake_padded;       // optional [2M] unscaled C2R output
+                              // (getPaddedWakePotential)
     double* filling;          // optional [nb+1] FP64 integral (+ total)
     const float2* Zc;         // CSR impedance [2M]
     const float* renorm;      // CSR renormalisation [2M]

@@ -225,6 +225,11 @@ __device__ void field_body(const FieldParams& p, uint32_t block_bunch,
     }
     VFPS_FSTAMP(4);
     const double* yd = reinterpret_cast<const double*>(A);
+    if (p.wake_padded) {
+        for (uint32_t i = threadIdx.x; i < 2 * M; i += blockDim.x) {
+            p.wake_padded[i] = (float)yd[i];
+        }
+    }
     for (uint32_t b = 0; b < p.nb; b++) {
         const uint32_t base = p.buckets[b] * p.spacing;
         for (uint32_t x = threadIdx.x; x < p.n; x += blockDim.x) {

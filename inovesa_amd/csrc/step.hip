@@ -457,6 +457,8 @@ static int step_upload_desc(vfps_step* s)
     d.counter = s->counter;
     d.err = s->err;
     d.seq_proj = s->desc.fast_projection ? 0u : 1u;
+    d.norm_every = s->desc.renormalize;
+    d.fill_set = t1->filling_set.empty() ? 1.0f : t1->filling_set[0];
     // the halo depends on the maps only: computed once
     if (!s->halo_valid) {
         if (s->desc.scheme == 0) {
@@ -743,6 +745,9 @@ int vfps_step_create(vfps_dev* dev, const vfps_step_desc* desc,
     s->desc = *desc;
     s->fused = desc->fused != 0;
     s->direct = s->fused && desc->wake_method != 1;
+    VFPS_CHECK_ARG(desc->renormalize == 0 || s->direct,
+                   "vfps_step_create: charge renormalisation needs the fused "
+                   "step with the direct wake");
     s->zwake = zwake;
     s->zcsr = zcsr;
     const uint32_t n = desc->n;
@@ -1038,6 +1043,40 @@ int vfps_step_get_mesh(vfps_step* s, float* mesh, float* proj, float* wake)
         } else {
             std::fill(wake, wake + n, 0.0f);
         }
+    }
+    return VFPS_OK;
+}
+
+int vfps_step_moments(vfps_step* s, float* xproj, float* yproj,
+                      float* population, float* position, float* length,
+                      float* average, float* spread)
+{
+    VFPS_CHECK_ARG(s, "vfps_step_moments: NULL step");
+    VFPS_HIP(hipStreamSynchronize(s->dev->stream));
+    vfps_mesh* cur = (s->fused && (s->parity & 1u)) ? s->t2 : s->t1;
+    double fill = 0.0;
+    // src/main.cpp:961-967: integrate(); variance(0); updateYProjection();
+    // variance(1)
+    int rc = vfps_mesh_integrate(cur, &fill, nullptr);
+    if (rc != VFPS_OK) return rc;
+    float mean = 0.0f, var = 0.0f, rms = 0.0f;
+    if ((rc = vfps_mesh_variance(cur, 0, &mean, &var, &rms)) != VFPS_OK) {
+        return rc;
+    }
+    if (position) *position = mean;
+    if (length) *length = rms;
+    if ((rc = vfps_mesh_project_y(cur)) != VFPS_OK) return rc;
+    if ((rc = vfps_mesh_variance(cur, 1, &mean, &var, &rms)) != VFPS_OK) {
+        return rc;
+    }
+    if (average) *average = mean;
+    if (spread) *spread = rms;
+    if (population) *population = (float)fill;
+    if (xproj && (rc = vfps_mesh_get_projection(cur, 0, xproj)) != VFPS_OK) {
+        return rc;
+    }
+    if (yproj && (rc = vfps_mesh_get_projection(cur, 1, yproj)) != VFPS_OK) {
+        return rc;
     }
     return VFPS_OK;
 }

@@ -46,6 +46,10 @@ struct InstDesc {
                                   // the field kernel
     uint32_t* err;                // set if a tap falls outside the halo
     uint32_t seq_proj;            // 1: x projection in the reference's order
+    uint32_t norm_every;          // > 0: renormalise the charge before the
+                                  // maps of every step whose count is a
+                                  // multiple (src/main.cpp:953-955)
+    float fill_set;               // the set bunch population
     unsigned long long* tstamp;   // optional timestamps (phases of tile 0,
                                   // entry/exit of every workgroup)
 };

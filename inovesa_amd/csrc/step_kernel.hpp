@@ -309,6 +309,7 @@ __device__ __forceinline__ void kick2_rows(const float* __restrict__ src,
 struct StepShared {
     double red[16];
     float s00;
+    float norm;   // renormalisation factor of this step (1: none)
     uint32_t slot;
     uint32_t cnt;
 };
@@ -653,6 +654,35 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
     }
     __syncthreads();
     const uint32_t cnt = cnt_sh;
+    // PhaseSpace::integrateAndNormalize (src/PS/PhaseSpace.cpp:342-372) on
+    // the steps the reference renormalises: the charge of the profile that
+    // entered the step (FP64, summed by wave 0 in a fixed order, so every
+    // tile of the instance finds the same factor), then every input sample
+    // times fill_set/filling before the maps.  The profile itself stays,
+    // as in the reference.
+    const bool renorm = DIRECT && d.norm_every != 0 && cnt % d.norm_every == 0;
+    float norm = 1.0f;
+    if (renorm) {
+        if (wave == 0) {
+            double f = 0.0;
+            for (uint32_t x = lane; x < n; x += 64) {
+                f += (double)pprev[x] * (double)ws_g[x];
+            }
+            f = wave_sum_dpp(f);
+            if (lane == 0) {
+                sh.norm = d.fill_set > 0.0f ? d.fill_set / (float)f : 0.0f;
+            }
+        }
+        __syncthreads();
+        norm = sh.norm;
+        if (need_row0) {
+            float* row0 = lds + L.row0;
+            for (uint32_t y = tid; y < n; y += bs) {
+                row0[y] = row0[y] * norm;
+            }
+        }
+        __syncthreads();
+    }
     VFPS_STAMP(1);
 
     // ---- B: direct wake offsets and the CSR share --------------------------
@@ -731,6 +761,15 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
         fill_rows();
     }
     __syncthreads();
+    if (renorm) {
+        // the buffer rows of the mesh, rescaled before any map reads them
+        float* dst = bufA + (size_t)lr0 * n;
+        const uint32_t tot = (uint32_t)(lr1 - lr0) * n;
+        for (uint32_t i = tid; i < tot; i += bs) {
+            dst[i] = dst[i] * norm;
+        }
+        __syncthreads();
+    }
     if (DIRECT && csr && tid == 0) {
         for (int o = 0; o < rows_out; o++) {
             csr_q += csr_rows[o];
@@ -840,7 +879,7 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                 s00 = v;
             }
         } else {
-            s00 = in[0];
+            s00 = in[0] * norm;
         }
         VFPS_STAMP(4);
         const RotGeom g = d.rot;

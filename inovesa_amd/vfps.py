@@ -516,6 +516,11 @@ class ElectricField:
         call("vfps_field_get_padded", self.h, _p(out))
         return out
 
+    def getPaddedWakePotential(self):
+        out = np.empty(self.nmax, np.float32)
+        call("vfps_field_get_padded_wake", self.h, _p(out))
+        return out
+
     def wakePotential(self):
         out = np.empty((self.ps.nb, self.ps.n), np.float32)
         call("vfps_field_wake_potential", self.h, _p(out))
@@ -647,6 +652,21 @@ class Step:
         w = np.empty(self.n, np.float32)
         call("vfps_step_get_mesh", self.h, _p(m), _p(p), _p(w))
         return m, p, w
+
+    def moments(self):
+        """Outstep quantities of the current state (src/main.cpp:961-967):
+        dict of xproj, yproj (n each) and population, position, length,
+        average, spread (floats)."""
+        n = self.n
+        xp = np.empty(n, np.float32)
+        yp = np.empty(n, np.float32)
+        v = [C.c_float() for _ in range(5)]
+        call("vfps_step_moments", self.h, _p(xp), _p(yp),
+             *[C.byref(x) for x in v])
+        names = ("population", "position", "length", "average", "spread")
+        out = {k: np.float32(x.value) for k, x in zip(names, v)}
+        out.update(xproj=xp, yproj=yp)
+        return out
 
     def set_mesh(self, mesh):
         m = _f32(mesh).reshape(self.n, self.n)

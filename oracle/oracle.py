@@ -77,7 +77,8 @@ class Oracle:
                         ("use_fp", _i32), ("fptype", _u32), ("e1", R),
                         ("deriv", _u32), ("use_wake", _i32), ("nmax", _u32),
                         ("spacing_bins", _u32), ("wakescaling_arg", R),
-                        ("use_csr", _i32), ("fc", R)]
+                        ("use_csr", _i32), ("fc", R),
+                        ("renormalize", _u32)]
         self.Cfg = Cfg
         L = self.lib
         L.or_sim_create.restype = _vp

@@ -242,6 +242,9 @@ typedef struct {
     real wakescaling_arg;
     int32_t use_csr;       /* compute CSR power every step */
     real fc;
+    uint32_t renormalize;  /* RenormalizeCharge: integrateAndNormalize on
+                              every step whose count is a multiple
+                              (src/main.cpp:953-955); 0: never */
 } or_sim_cfg;
 
 or_sim* or_sim_create(const or_sim_cfg* cfg, const real* init,

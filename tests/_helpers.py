@@ -5,7 +5,7 @@ ORACLE_KEYS = ("n", "qmin", "qmax", "pmin", "pmax", "qscale", "pscale",
                "scheme", "interp", "rot_clamp", "angle", "rf_linear",
                "revolutionpart", "V_RF", "f_RF", "V0", "slip", "E0", "use_fp",
                "fptype", "e1", "deriv", "use_wake", "spacing_bins", "use_csr",
-               "fc")
+               "fc", "renormalize")
 
 
 def oracle_cfg(desc, nmax):
