@@ -208,6 +208,17 @@ VFPS_API int vfps_map_create_wakepotential(vfps_mesh* in, vfps_mesh* out,
  * potential of the field's phase space becomes this map's offset, on the
  * device */
 VFPS_API int vfps_map_update(vfps_map* map);
+/* SourceMap::applyToAll (src/SM/SourceMap.cpp:111-117): the map's applyTo
+ * (KickMap src/SM/KickMap.cpp:280-304, RotationMap
+ * src/SM/RotationMap.cpp:178-188, FokkerPlanckMap with the FPTracking the
+ * map was built with, src/SM/FokkerPlanckMap.cpp:214-268; Identity moves
+ * nothing) on n particles in grid coordinates ([n][2]: x, y; host array,
+ * updated in place), on the device; synchronises.  The stochastic
+ * Fokker-Planck tracking draws its noise from a counter-based generator
+ * (seed, particle, call) where the reference uses std::mt19937 seeded from
+ * std::random_device. */
+VFPS_API int vfps_map_apply_to(vfps_map* map, float* pos, uint32_t n,
+                               uint64_t seed);
 /* SourceMap::apply (src/SM/SourceMap.cpp:75-109 and overrides) */
 VFPS_API int vfps_map_apply(vfps_map* map);
 VFPS_API int vfps_map_destroy(vfps_map* map);
@@ -433,6 +444,15 @@ VFPS_API int vfps_step_get_history(vfps_step* s, uint32_t* count,
 VFPS_API int vfps_step_get_mesh(vfps_step* s, float* mesh, float* proj,
                                 float* wake);
 VFPS_API int vfps_step_set_mesh(vfps_step* s, const float* mesh);
+/* Particles tracked through every following step (src/main.cpp:1034-1041:
+ * applyToAll after each map's apply), n positions in grid coordinates
+ * ([n][2]), FokkerPlanckMap tracking fptrack (FPTracking 0-3; 2 needs the
+ * per-map step, fused = 0), noise seed; n = 0 stops tracking.  The tracking
+ * runs on the device beside the step (inside its captured graphs). */
+VFPS_API int vfps_step_set_particles(vfps_step* s, const float* pos,
+                                     uint32_t n, int fptrack, uint64_t seed);
+/* the tracked positions ([n][2], n as set); synchronises */
+VFPS_API int vfps_step_get_particles(vfps_step* s, float* pos, uint32_t n);
 /* The outstep moments of the current state (src/main.cpp:961-967:
  * integrate(), variance(0), updateYProjection(), variance(1)), as
  * HDF5File::append reads them (src/IO/HDF5File.cpp:402-418): x projection

@@ -173,6 +173,9 @@ _SIGS = {
     "vfps_step_get_mesh": (_int, [_vp, _vp, _vp, _vp]),
     "vfps_step_set_mesh": (_int, [_vp, _vp]),
     "vfps_step_moments": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "vfps_step_set_particles": (_int, [_vp, _vp, _u32, _int, C.c_uint64]),
+    "vfps_step_get_particles": (_int, [_vp, _vp, _u32]),
+    "vfps_map_apply_to": (_int, [_vp, _vp, _u32, C.c_uint64]),
     "vfps_step_timing": (_int, [_vp, _u32, _vp, _vp, _vp]),
     "vfps_step_phases": (_int, [_vp, _vp, _u32]),
     "vfps_step_config": (_int, [_vp, _vp, _u32]),

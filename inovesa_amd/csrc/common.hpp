@@ -111,6 +111,12 @@ struct vfps_map {
     int ip = 0;
     float* fp_w = nullptr;      // [ip][n]
     uint32_t* fp_i = nullptr;   // [ip][n]
+    // FokkerPlanckMap particle tracking (FPTracking, src/SM/FokkerPlanckMap.cpp
+    // :214-268): 0 none, 1 approximation1, 2 approximation2, 3 stochastic
+    int fptrack = 0;
+    float fp_e1 = 0;
+    uint64_t track_seed = 0;   // stochastic tracking: generator seed
+    uint32_t track_calls = 0;  // applyTo calls so far (generator counter)
     // DynamicRFKickMap: RF kick offsets recomputed before every apply from
     // the next (phase, amplitude) of a modulation queue built at
     // construction

@@ -628,7 +628,7 @@ int vfps_map_create_fokkerplanck(vfps_mesh* in, vfps_mesh* out, int fptype,
                                  int fptrack, float e1, int deriv,
                                  vfps_map** map)
 {
-    (void)fptrack;  // particle tracking is not part of the mesh update
+    // particle tracking (applyTo) reads the mode and e1 from the map
     int rc = check_pair(in, out);
     if (rc != VFPS_OK) return rc;
     VFPS_CHECK_ARG(map, "FokkerPlanckMap: NULL output");
@@ -650,11 +650,14 @@ int vfps_map_create_fokkerplanck(vfps_mesh* in, vfps_mesh* out, int fptype,
             wt[(size_t)j * n + y] = w[(size_t)y * ip + j];
         }
     }
+    VFPS_CHECK_ARG(fptrack >= 0 && fptrack <= 3, "FokkerPlanckMap: FPTracking");
     auto* m = new vfps_map();
     m->kind = MAP_FP;
     m->in = in;
     m->out = out;
     m->ip = ip;
+    m->fptrack = fptrack;
+    m->fp_e1 = e1;
     hipError_t e = hipMalloc(&m->fp_w, sizeof(float) * wt.size());
     if (e == hipSuccess) e = hipMalloc(&m->fp_i, sizeof(uint32_t) * ixt.size());
     if (e != hipSuccess) {

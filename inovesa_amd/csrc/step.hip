@@ -345,7 +345,21 @@ static int step_launch_fused(vfps_step* s, uint32_t parity, hipStream_t st)
                               s->threads, s->desc.n, parity, st, &s->inst_h);
 }
 
+static int step_launch_maps(vfps_step* s, uint32_t parity);
+
+// one step, then the tracked particles through its maps (applyToAll after
+// each apply, src/main.cpp:1034-1041); the step counter the noise of the
+// stochastic tracking draws on is the one the step has just written
 static int step_launch_one(vfps_step* s, uint32_t parity)
+{
+    int rc = step_launch_maps(s, parity);
+    if (rc != VFPS_OK || s->npart == 0) return rc;
+    const uint32_t* cnt = s->counter + (s->fused ? (parity ^ 1u) : 0u);
+    return launch_track(s->track, s->particles, s->npart, cnt, 0,
+                        s->dev->stream);
+}
+
+static int step_launch_maps(vfps_step* s, uint32_t parity)
 {
     hipStream_t st = s->dev->stream;
     const bool wake = s->desc.use_wake != 0;
@@ -396,6 +410,7 @@ static void step_free(vfps_step* s)
         hipGraphDestroy(kv.second.first);
     }
     s->graphs.clear();
+    hipFree(s->particles);
     vfps_map_destroy(s->wm);
     vfps_map_destroy(s->rfm);
     vfps_map_destroy(s->drm);
@@ -1043,6 +1058,64 @@ int vfps_step_get_mesh(vfps_step* s, float* mesh, float* proj, float* wake)
         } else {
             std::fill(wake, wake + n, 0.0f);
         }
+    }
+    return VFPS_OK;
+}
+
+int vfps_step_set_particles(vfps_step* s, const float* pos, uint32_t n,
+                            int fptrack, uint64_t seed)
+{
+    VFPS_CHECK_ARG(s && (pos || n == 0), "vfps_step_set_particles: NULL "
+                                         "argument");
+    VFPS_CHECK_ARG(fptrack >= 0 && fptrack <= 3,
+                   "vfps_step_set_particles: FPTracking");
+    VFPS_CHECK_ARG(!(s->fused && fptrack == 2),
+                   "vfps_step_set_particles: FPTracking approximation2 reads "
+                   "the mesh between drift and Fokker-Planck, which only the "
+                   "per-map step (fused = 0) keeps");
+    VFPS_TRY_BEGIN
+    VFPS_HIP(hipStreamSynchronize(s->dev->stream));
+    hipFree(s->particles);
+    s->particles = nullptr;
+    s->npart = 0;
+    TrackProgram prog{};
+    prog.seed = seed;
+    auto add = [&](const vfps_map* m, const float* off) {
+        if (m && track_op(m, off, &prog.op[prog.nops])) {
+            prog.nops++;
+        }
+    };
+    if (s->desc.use_wake) add(s->wm, s->wfield->wake);
+    if (s->desc.scheme == 0) {
+        add(s->rfm, nullptr);
+        add(s->drm, nullptr);
+    } else {
+        add(s->rot, nullptr);
+    }
+    if (s->fpm) {
+        add(s->fpm, nullptr);
+        prog.op[prog.nops - 1].fptrack = fptrack;
+    }
+    s->track = prog;
+    if (n) {
+        VFPS_HIP(hipMalloc(&s->particles, sizeof(float2) * n));
+        VFPS_HIP(hipMemcpy(s->particles, pos, sizeof(float2) * n,
+                           hipMemcpyHostToDevice));
+        s->npart = n;
+    }
+    // the captured steps gain (or lose) the tracking launch
+    return step_drop_graph(s);
+    VFPS_TRY_END
+}
+
+int vfps_step_get_particles(vfps_step* s, float* pos, uint32_t n)
+{
+    VFPS_CHECK_ARG(s && pos && n == s->npart,
+                   "vfps_step_get_particles: bad argument");
+    VFPS_HIP(hipStreamSynchronize(s->dev->stream));
+    if (n) {
+        VFPS_HIP(hipMemcpy(pos, s->particles, sizeof(float2) * n,
+                           hipMemcpyDeviceToHost));
     }
     return VFPS_OK;
 }

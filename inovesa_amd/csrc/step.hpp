@@ -8,6 +8,7 @@
 #include "common.hpp"
 #include "field.hpp"
 #include "maps.hpp"
+#include "track.hpp"
 
 namespace vfps_impl {
 
@@ -149,6 +150,10 @@ struct vfps_step {
     bool in_batch = false; // a batch member (the batch forms' LDS layout)
     bool fused = true;
     bool direct = true;
+    // passive particle tracking through every step's maps
+    float2* particles = nullptr;
+    uint32_t npart = 0;
+    vfps_impl::TrackProgram track{};
     uint32_t parity = 0;  // fused path: buffer holding the state (0: t1)
     // captured runs of k steps from parity p, key k << 1 | p
     std::map<uint32_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs;

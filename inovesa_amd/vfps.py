@@ -215,6 +215,13 @@ class SourceMap:
     def apply(self):
         call("vfps_map_apply", self.h)
 
+    def applyToAll(self, positions, seed=0):
+        """SourceMap::applyToAll: the map's applyTo on [n][2] grid
+        coordinates (x, y); returns the moved positions."""
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 2).copy()
+        call("vfps_map_apply_to", self.h, _p(p), p.shape[0], seed)
+        return p
+
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
@@ -652,6 +659,18 @@ class Step:
         w = np.empty(self.n, np.float32)
         call("vfps_step_get_mesh", self.h, _p(m), _p(p), _p(w))
         return m, p, w
+
+    def set_particles(self, positions, fptrack=3, seed=0):
+        """Track [n][2] grid coordinates through every following step."""
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 2)
+        call("vfps_step_set_particles", self.h, _p(p), p.shape[0], fptrack,
+             seed)
+        self.npart = p.shape[0]
+
+    def particles(self):
+        out = np.empty((getattr(self, "npart", 0), 2), np.float32)
+        call("vfps_step_get_particles", self.h, _p(out), out.shape[0])
+        return out
 
     def moments(self):
         """Outstep quantities of the current state (src/main.cpp:961-967):

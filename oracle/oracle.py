@@ -114,6 +114,9 @@ class Oracle:
         L.or_rfkick_offsets.argtypes = [_u32, _vp, R, R, R, R, R, R, R, _vp]
         L.or_rfkick_offsets_at.argtypes = [_u32, _vp, R, R, R, R, R, R, R,
                                            R, R, _vp]
+        L.or_track_kick.argtypes = [C.c_int, _u32, _u32, _vp, _vp]
+        L.or_track_rotation.argtypes = [R, R, R, R, _vp]
+        L.or_track_fp.argtypes = [C.c_int, _u32, _u32, _u32, _vp, _vp, _vp]
         L.or_rfkick_offsets_linear_at.argtypes = [_u32, R, R, R, R, R, R, R,
                                                   R, _vp]
         L.or_rotation_hinfo.argtypes = [_u32, _u32, _vp, R, R, _vp, R, R, R,
@@ -306,6 +309,29 @@ class Oracle:
         self.lib.or_rfkick_offsets_linear_at(n, zerobin0, angle, qscale, f_RF,
                                              delta0, syncphase, phase, ampl,
                                              _ptr(out))
+        return out
+
+    def track_kick(self, axis, n, offset, pos):
+        """KickMap::applyTo on each [x, y] of pos (src/SM/KickMap.cpp:280-304)."""
+        off = self.arr(offset)
+        out = np.array(pos, self.real).reshape(-1, 2).copy()
+        for p in out:
+            self.lib.or_track_kick(axis, n, n, _ptr(off), _ptr(p))
+        return out
+
+    def track_rotation(self, cos_dt, sin_dt, zb0, zb1, pos):
+        out = np.array(pos, self.real).reshape(-1, 2).copy()
+        for p in out:
+            self.lib.or_track_rotation(cos_dt, sin_dt, zb0, zb1, _ptr(p))
+        return out
+
+    def track_fp(self, mode, hinfo, ip, n, data_in, pos):
+        """FokkerPlanckMap::applyTo approximation1/2 (mode 1/2)."""
+        out = np.array(pos, self.real).reshape(-1, 2).copy()
+        d = self.arr(data_in) if data_in is not None else None
+        for p in out:
+            self.lib.or_track_fp(mode, ip, n, n, _ptr(hinfo), _ptr(d),
+                                 _ptr(p))
         return out
 
     @staticmethod

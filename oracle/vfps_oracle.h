@@ -262,4 +262,12 @@ void or_sim_set_mesh(or_sim* s, const real* mesh);
 }
 #endif
 
+/* ---- passive particle tracking: SourceMap::applyTo (pos = {x, y}) ---- */
+void or_track_kick(int axis, uint32_t pd, uint32_t kd, const real* offset,
+                   real* pos);
+void or_track_rotation(real cos_dt, real sin_dt, real zb0, real zb1,
+                       real* pos);
+void or_track_fp(int mode, uint32_t ip, uint32_t ysize, uint32_t nx,
+                 const or_hi* hinfo, const real* data_in, real* pos);
+
 #endif
