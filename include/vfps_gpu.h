@@ -335,6 +335,8 @@ typedef struct {
     double rotations;
     double bunch_current;     /* single bunch */
     double cutoff_frequency;  /* fc */
+    double steps_per_trev;    /* StepsPerRevolution; > 0 overrides
+                                 steps_per_ts (src/main.cpp:284-286) */
 } vfps_ring_params;
 
 typedef struct {

@@ -94,6 +94,13 @@ VFPS_H5_API int vfps_h5_add_parameter_i64(vfps_h5* f, const char* group,
 VFPS_H5_API int vfps_h5_add_parameter_str(vfps_h5* f, const char* group,
                                           const char* name,
                                           const char* value);
+/* addParameterToGroup with the data type ProgramOptions::save(HDF5File*)
+ * (src/IO/ProgramOptions.cpp:460-490) gives each option: value points to a
+ * float, double, uint32_t or int32_t */
+enum { VFPS_H5_F32 = 0, VFPS_H5_F64 = 1, VFPS_H5_U32 = 2, VFPS_H5_I32 = 3 };
+VFPS_H5_API int vfps_h5_add_parameter_typed(vfps_h5* f, const char* group,
+                                            const char* name, int type,
+                                            const void* value);
 /* number of rows appended so far to the named data set (e.g.
  * "/PhaseSpace/data") */
 VFPS_H5_API int vfps_h5_rows(vfps_h5* f, const char* dataset,

@@ -50,7 +50,7 @@ class RingParams(C.Structure):
                 ("damping_time", _f64), ("padding", _f64),
                 ("round_padding", _i32), ("steps_per_ts", _u32),
                 ("rotations", _f64), ("bunch_current", _f64),
-                ("cutoff_frequency", _f64)]
+                ("cutoff_frequency", _f64), ("steps_per_trev", _f64)]
 
 
 class Derived(C.Structure):

@@ -77,7 +77,9 @@ extern "C" int vfps_derive(const vfps_ring_params* p, vfps_derived* d)
                       / V_eff * fs;
     const double Ib = (double)(float)p->bunch_current;
     // src/main.cpp:284-297
-    const double steps = std::max(p->steps_per_ts, 1u);
+    const double steps = (p->steps_per_trev > 0)
+                             ? p->steps_per_trev * f_rev / fs
+                             : std::max(p->steps_per_ts, 1u);
     const float rotations = (float)p->rotations;
     const double calc_damp = E0 * E_CHARGE / W0 / f_rev;
     const double t_damp = (p->damping_time < 0) ? calc_damp : p->damping_time;

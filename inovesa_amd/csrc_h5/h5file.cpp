@@ -116,6 +116,17 @@ bool attr_str(hid_t obj, const char* name, const char* v)
     return ok;
 }
 
+bool attr_typed(hid_t obj, const char* name, hid_t ftype, hid_t mtype,
+                const void* v)
+{
+    const hid_t sp = H5Screate(H5S_SCALAR);
+    const hid_t a = H5Acreate2(obj, name, ftype, sp, H5P_DEFAULT, H5P_DEFAULT);
+    bool ok = a >= 0 && H5Awrite(a, mtype, v) >= 0;
+    if (a >= 0) H5Aclose(a);
+    H5Sclose(sp);
+    return ok;
+}
+
 bool ds_attr(vfps_h5* f, const std::string& ds, const char* name, double v)
 {
     return attr_f64(f->ds.at(ds).id, name, v);
@@ -480,6 +491,27 @@ int vfps_h5_add_parameter_str(vfps_h5* f, const char* group, const char* name,
                               const char* value)
 {
     return add_parameter(f, group, name, 2, 0.0, 0, value);
+}
+
+int vfps_h5_add_parameter_typed(vfps_h5* f, const char* group,
+                                const char* name, int type, const void* value)
+{
+    if (!f || !group || !name || !value) {
+        return fail("addParameterToGroup: bad argument");
+    }
+    hid_t ft, mt;
+    switch (type) {
+    case VFPS_H5_F32: ft = H5T_IEEE_F32LE; mt = H5T_NATIVE_FLOAT; break;
+    case VFPS_H5_F64: ft = H5T_IEEE_F64LE; mt = H5T_NATIVE_DOUBLE; break;
+    case VFPS_H5_U32: ft = H5T_STD_U32LE; mt = H5T_NATIVE_UINT32; break;
+    case VFPS_H5_I32: ft = H5T_STD_I32LE; mt = H5T_NATIVE_INT32; break;
+    default: return fail("addParameterToGroup: type");
+    }
+    const hid_t g = H5Gopen2(f->file, group, H5P_DEFAULT);
+    if (g < 0) return fail(std::string("addParameterToGroup: no group ") + group);
+    const bool ok = attr_typed(g, name, ft, mt, value);
+    H5Gclose(g);
+    return ok ? 0 : fail(std::string("addParameterToGroup: ") + name);
 }
 
 int vfps_h5_rows(vfps_h5* f, const char* dataset, uint64_t* rows)

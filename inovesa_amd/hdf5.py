@@ -52,6 +52,8 @@ _SIGS = {
                                          C.c_int64]),
     "vfps_h5_add_parameter_str": (_int, [_vp, C.c_char_p, C.c_char_p,
                                          C.c_char_p]),
+    "vfps_h5_add_parameter_typed": (_int, [_vp, C.c_char_p, C.c_char_p,
+                                           _int, _vp]),
     "vfps_h5_rows": (_int, [_vp, C.c_char_p, C.POINTER(C.c_uint64)]),
     "vfps_h5_read_phasespace": (_int, [C.c_char_p, C.c_int64,
                                        C.POINTER(_u32), C.POINTER(_u32),
@@ -182,6 +184,17 @@ class HDF5File:
         else:
             rc = lib().vfps_h5_add_parameter_f64(self.h, g, k, float(value))
         _check(rc)
+
+    def addTypedParameter(self, group, name, ctype, value):
+        """An attribute of the HDF5 type of ctype (float, double, uint32 or
+        int32), as ProgramOptions::save(HDF5File*) writes the options."""
+        kinds = {"float": (0, C.c_float), "double": (1, C.c_double),
+                 "uint32": (2, C.c_uint32), "int32": (3, C.c_int32)}
+        kind, ct = kinds[ctype]
+        v = ct(value)
+        _check(lib().vfps_h5_add_parameter_typed(self.h, group.encode(),
+                                                 name.encode(), kind,
+                                                 C.byref(v)))
 
     def rows(self, dataset):
         r = C.c_uint64()

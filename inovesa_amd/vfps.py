@@ -568,7 +568,7 @@ _RING_DEFAULTS = dict(  # src/IO/ProgramOptions.cpp:13-305
     f_rev=9e6, harmonic_number=50.0, rf_voltage=1e6, f_s=0.0, alpha0=4e-3,
     alpha1=0.0, alpha2=0.0, vacuum_gap=0.03, damping_time=-1.0, padding=8.0,
     round_padding=1, steps_per_ts=1000, rotations=5.0, bunch_current=3e-3,
-    cutoff_frequency=23e9)
+    cutoff_frequency=23e9, steps_per_trev=0.0)
 
 
 def derive(**ring):
