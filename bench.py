@@ -18,6 +18,8 @@ Further keys of the line:
                 mesh traffic per cell), timed with HIP events on its stream;
   fft_wake      the same workload with the wake and CSR power from the FFT
                 field kernel;
+  per_map       the same workload one kernel per map (the per-map path an
+                unchanged reference loop drives);
   batched       64 independent meshes of 128^2-512^2 (kick-drift maps), one
                 launch per step and size class;
   batched_rotation  the same 64 meshes on the rotation maps (wake kick,
@@ -41,6 +43,18 @@ sys.path.insert(0, ROOT)
 METRIC = ("mesh-cell updates/s at 512²; batched+surrogate threshold scan "
           "time vs plain @2%")
 HBM_PEAK_GBS = 8000.0
+
+
+# tilings the autotuner chose on the MI355X for the batch lines (mesh size:
+# rows per tile, workgroup size, persistent), used by --only so that a
+# profiler run times the same kernels without the tuner's launches
+PROFILE_TILES = {
+    "batched": {128: (2, 256, False), 256: (4, 256, False),
+                384: (8, 512, False), 512: (8, 512, False)},
+    "batched_rotation": {128: (2, 256, False), 256: (4, 256, False),
+                         384: (8, 512, False), 512: (8, 512, False)},
+    "production": (4, 512, False),
+}
 
 
 def host_cpu():
@@ -108,7 +122,8 @@ def launch_label(steps, no_graph):
             "then one finalize launch")
 
 
-def batched_throughput(steps, log, scheme="kick_drift"):
+def batched_throughput(steps, log, scheme="kick_drift", tiles=None,
+                       no_graph=False):
     """64 independent meshes of uneven sizes (16 each of 128^2, 256^2,
     384^2, 512^2; currents log-spaced over 0.3-3x threshold; gaps 16-64 mm,
     each its own parallel-plates impedance) advanced together after the
@@ -121,9 +136,16 @@ def batched_throughput(steps, log, scheme="kick_drift"):
     from inovesa_amd.workloads import batched_scan_config, impedances
     cfg = batched_scan_config(scheme=scheme)
     zs = [impedances(d) for _, d, _ in cfg]
-    b = Batch([c[0] for c in cfg], [c[2] for c in cfg], [z[0] for z in zs],
-              [z[1] for z in zs])
-    table = b.autotune(steps=10)
+    b = Batch([dict(c[0], no_graph=int(no_graph)) for c in cfg],
+              [c[2] for c in cfg], [z[0] for z in zs], [z[1] for z in zs])
+    if tiles:
+        for size, (T, thr, persistent) in tiles.items():
+            b.retile_size(size, T, thr, persistent=persistent)
+        table = [dict(n=size, T=T, threads=thr, persistent=persistent,
+                      us_per_step=0.0, chosen=True)
+                 for size, (T, thr, persistent) in tiles.items()]
+    else:
+        table = b.autotune(steps=10)
     cells = b.info()["cells"]
     b.record(steps + 128)
     b.run(128)
@@ -144,11 +166,11 @@ def batched_throughput(steps, log, scheme="kick_drift"):
             "cells_per_step": int(cells),
             "steps": steps, "ms_per_step": dt * 1e3 / steps,
             "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
-            "launch": launch_label(steps, False),
+            "launch": launch_label(steps, no_graph),
             "tiles_chosen": {str(k): v for k, v in chosen.items()}}
 
 
-def production_throughput(steps, log):
+def production_throughput(steps, log, tiles=None, no_graph=False):
     """The large production workload: 16 independent 1024^2 meshes
     (second-order momentum compaction, parallel-plates CSR plus resistive
     wall, five-point Fokker-Planck, 4000 steps per period; currents
@@ -163,12 +185,18 @@ def production_throughput(steps, log):
                                    np.linspace(0.024, 0.056, 16))):
         desc, d, rw = production_config(gap=float(g), current_factor=float(f))
         zw, zc = impedances(d, resistive_wall=rw)
-        descs.append(desc)
+        descs.append(dict(desc, no_graph=int(no_graph)))
         inits.append(initial_mesh(desc["n"], seed=100 + k))
         zws.append(zw)
         zcs.append(zc)
     b = Batch(descs, inits, zws, zcs)
-    table = b.autotune(steps=6)
+    if tiles:
+        T, thr, persistent = tiles
+        b.retile_size(1024, T, thr, persistent=persistent)
+        table = [dict(n=1024, T=T, threads=thr, persistent=persistent,
+                      us_per_step=0.0, chosen=True)]
+    else:
+        table = b.autotune(steps=6)
     cells = b.info()["cells"]
     b.record(steps + 64)
     b.run(64)
@@ -186,7 +214,7 @@ def production_throughput(steps, log):
             "instances": 16, "n": 1024, "cells_per_step": int(cells),
             "steps": steps, "ms_per_step": dt * 1e3 / steps,
             "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
-            "launch": launch_label(steps, False),
+            "launch": launch_label(steps, no_graph),
             "tiles_chosen": chosen[0] if chosen else None}
 
 
@@ -245,10 +273,28 @@ def main():
     ap.add_argument("--scan-targets", type=int, default=128)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB and exit")
+    ap.add_argument("--only", choices=("batched", "batched_rotation",
+                                       "production"),
+                    help="run only this batch line, with the tiling the "
+                         "autotuner last chose (for profiler runs), and "
+                         "print its JSON object")
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.only:
+        if args.only == "production":
+            r = production_throughput(args.batch_steps, log,
+                                      tiles=PROFILE_TILES["production"],
+                                      no_graph=args.no_graph)
+        else:
+            r = batched_throughput(args.batch_steps, log,
+                                   scheme="rotation" if args.only ==
+                                   "batched_rotation" else "kick_drift",
+                                   tiles=PROFILE_TILES[args.only],
+                                   no_graph=args.no_graph)
+        print(json.dumps({args.only: r}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: the step runs on one GPU; --gpus {args.gpus} is "
@@ -328,6 +374,25 @@ def main():
                 "ms_per_step": dtf * 1e3 / kf, "field_kernel_ms": ms_field,
                 "step_kernel_ms": ms_maps_fft}
 
+    # the per-map path: one kernel per map and the FFT field, as an
+    # unchanged reference loop calls it (SourceMap::apply per map)
+    pm = Step(dict(desc, fused=0, no_graph=int(args.no_graph)), init, zw, zc)
+    pm.run(64)
+    kp = min(args.steps, 2000)
+    pm.prepare(kp)
+    pm.dev.sync()
+    t0 = time.perf_counter()
+    pm.run(kp)
+    pm.dev.sync()
+    dtp = time.perf_counter() - t0
+    per_map_line = {"value": n * n * kp / dtp, "steps": kp,
+                    "ms_per_step": dtp * 1e3 / kp,
+                    "roofline_frac": 8.0 * n * n * kp / dtp
+                    / (HBM_PEAK_GBS * 1e9),
+                    "kernels": "field (FFT wake + CSR), wake kick, RF kick, "
+                               "drift, Fokker-Planck, x projection"}
+    del pm
+
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
@@ -346,7 +411,7 @@ def main():
                       "tile_rows": cfg["tile_rows"],
                       "threads": cfg["threads"]},
            "roofline": roofline, "fft_wake": fft_line,
-           "other_projection": other_line}
+           "other_projection": other_line, "per_map": per_map_line}
     if not args.no_batch:
         out["batched"] = batched_throughput(args.batch_steps, log)
         out["batched_rotation"] = batched_throughput(args.batch_steps, log,

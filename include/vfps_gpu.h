@@ -501,7 +501,8 @@ VFPS_API int vfps_batch_create(vfps_dev* dev, uint32_t ninst,
                                vfps_impedance* const* zcsr, vfps_batch** out);
 VFPS_API int vfps_batch_destroy(vfps_batch* b);
 /* nsteps steps of every instance, replayed from captured graphs as in
- * vfps_step_run */
+ * vfps_step_run (launched from the host when the first instance's
+ * description sets no_graph) */
 VFPS_API int vfps_batch_run(vfps_batch* b, uint32_t nsteps);
 /* the graphs vfps_batch_run(b, nsteps) replays, captured ahead */
 VFPS_API int vfps_batch_prepare(vfps_batch* b, uint32_t nsteps);

@@ -326,6 +326,13 @@ int vfps_batch_run(vfps_batch* b, uint32_t nsteps)
     hipStream_t st = b->dev->stream;
     uint32_t left = nsteps;
     int rc;
+    // host launches when the members ask for them (no_graph)
+    const bool host = !b->members.empty() && b->members[0]->desc.no_graph;
+    while (host && left > 0) {
+        if ((rc = launch_step(b, b->parity)) != VFPS_OK) return rc;
+        b->parity ^= 1u;
+        left--;
+    }
     // captured runs: full ones of BATCH_GRAPH steps (even, so the parity
     // stays), then one of the remaining steps
     while (left > 0) {

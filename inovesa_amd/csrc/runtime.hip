@@ -1,4 +1,5 @@
 // Device handle, errors and the PhaseSpace object (inc/PS/PhaseSpace.hpp).
+#include <algorithm>
 #include <cmath>
 #include <numeric>
 #include <stdexcept>
@@ -85,40 +86,130 @@ __global__ void outer_kernel(uint32_t nb, uint32_t n, const float* g,
     data[i] = g[x] * g[n + y];
 }
 
-// updateXProjection (src/PS/PhaseSpace.cpp:291-323): one thread per row,
-// summing y = 0..n-1 in order like std::inner_product, so the projection
-// equals the reference's bit for bit (the wake potential amplifies the
-// last bits of the profile through the high-frequency impedance).
-__global__ void project_x_kernel(uint32_t rows, uint32_t n, const float* data,
-                                 const float* ws, float* proj)
+// updateXProjection (src/PS/PhaseSpace.cpp:291-323): std::inner_product
+// order (acc = acc + data*ws for y = 0..n-1, a float accumulator), so the
+// projection equals the reference's bit for bit (the wake potential
+// amplifies the last bits of the profile through the high-frequency
+// impedance).  A workgroup takes PX_ROWS rows: all its threads form the
+// products data*ws with coalesced loads into LDS (rows padded by 4 words),
+// then one lane per row adds its row in order from LDS.
+constexpr uint32_t PX_ROWS = 16;
+constexpr uint32_t PX_THREADS = 256;
+
+__global__ void __launch_bounds__(PX_THREADS)
+project_x_kernel(uint32_t rows, uint32_t n, uint32_t rows_per_wg,
+                 const float* __restrict__ data,
+                 const float* __restrict__ ws, float* __restrict__ proj)
 {
-    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= rows) {
-        return;
+    extern __shared__ __align__(16) float px[];   // rows_per_wg * (n + 4)
+    const uint32_t r0 = blockIdx.x * rows_per_wg;
+    const uint32_t nr = min(rows_per_wg, rows - r0);
+    const uint32_t stride = n + 4;
+    const float* src = data + (size_t)r0 * n;
+    if ((n & 3u) == 0u) {
+        const uint32_t n4 = n >> 2;
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        const float4* w4 = reinterpret_cast<const float4*>(ws);
+        for (uint32_t i = threadIdx.x; i < nr * n4; i += blockDim.x) {
+            const uint32_t r = i / n4, g = i - r * n4;
+            const float4 v = s4[i];
+            const float4 w = w4[g];
+            float4 o;
+            o.x = v.x * w.x;
+            o.y = v.y * w.y;
+            o.z = v.z * w.z;
+            o.w = v.w * w.w;
+            *reinterpret_cast<float4*>(px + r * stride + 4 * g) = o;
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < nr * n; i += blockDim.x) {
+            const uint32_t r = i / n, y = i - r * n;
+            px[r * stride + y] = src[i] * ws[y];
+        }
     }
-    const float* r = data + (size_t)row * n;
-    float acc = 0.0f;
-    for (uint32_t y = 0; y < n; y++) {
-        acc = acc + r[y] * ws[y];
+    __syncthreads();
+    if (threadIdx.x < nr) {
+        const float* p = px + threadIdx.x * stride;
+        float acc = 0.0f;
+        uint32_t y = 0;
+        if ((n & 15u) == 0u) {
+            // the next 16 products load while the current 16 are added
+            const float4* p4 = reinterpret_cast<const float4*>(p);
+            float4 q[4], nq[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) q[k] = p4[k];
+            for (; y < n; y += 16) {
+                const uint32_t yn = y + 16 < n ? y + 16 : y;
+#pragma unroll
+                for (int k = 0; k < 4; k++) nq[k] = p4[(yn >> 2) + k];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    acc = acc + q[k].x;
+                    acc = acc + q[k].y;
+                    acc = acc + q[k].z;
+                    acc = acc + q[k].w;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) q[k] = nq[k];
+            }
+        }
+        for (; y < n; y++) {
+            acc = acc + p[y];
+        }
+        proj[r0 + threadIdx.x] = acc;
     }
-    proj[row] = acc;
 }
 
-// updateYProjection (src/PS/PhaseSpace.cpp:325-340): one thread per column
-__global__ void project_y_kernel(uint32_t nb, uint32_t n, const float* data,
-                                 const float* ws, float* proj)
+// updateYProjection (src/PS/PhaseSpace.cpp:325-340): for each column y,
+// proj[y] = proj[y] + data[x][y]*ws[x] for x = 0..n-1 in order.  A
+// workgroup takes 64 columns of one bunch, lane y of wave 0 their sums;
+// the other waves stage chunks of PY_ROWS rows (the products) into a
+// double buffer in LDS, one chunk ahead of the sums.
+constexpr uint32_t PY_COLS = 64;
+constexpr uint32_t PY_ROWS = 64;
+constexpr uint32_t PY_THREADS = 256;
+
+__global__ void __launch_bounds__(PY_THREADS)
+project_y_kernel(uint32_t nb, uint32_t n, const float* __restrict__ data,
+                 const float* __restrict__ ws, float* __restrict__ proj)
 {
-    uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t b = blockIdx.y;
-    if (y >= n) {
-        return;
-    }
+    __shared__ float buf[2][PY_ROWS][PY_COLS + 1];
+    const uint32_t b = blockIdx.y;
+    const uint32_t c0 = blockIdx.x * PY_COLS;
+    const uint32_t ncol = min(PY_COLS, n - c0);
     const float* d = data + (size_t)b * n * n;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
+    const uint32_t nchunks = (n + PY_ROWS - 1) / PY_ROWS;
+    // stage chunk k by the threads [t0, t0 + nt)
+    auto stage = [&](uint32_t k, uint32_t t0, uint32_t nt) {
+        const uint32_t x0 = k * PY_ROWS;
+        const uint32_t nr = min(PY_ROWS, n - x0);
+        float(*dst)[PY_COLS + 1] = buf[k & 1u];
+        for (uint32_t i = tid - t0; i < nr * PY_COLS; i += nt) {
+            const uint32_t r = i / PY_COLS, c = i % PY_COLS;
+            if (c < ncol) {
+                dst[r][c] = d[(size_t)(x0 + r) * n + c0 + c] * ws[x0 + r];
+            }
+        }
+    };
+    stage(0, 0, blockDim.x);
+    __syncthreads();
     float acc = 0.0f;
-    for (uint32_t x = 0; x < n; x++) {
-        acc = acc + d[(size_t)x * n + y] * ws[x];
+    for (uint32_t k = 0; k < nchunks; k++) {
+        if (wave != 0) {
+            if (k + 1 < nchunks) stage(k + 1, 64, blockDim.x - 64);
+        } else if (tid < ncol) {
+            const uint32_t nr = min(PY_ROWS, n - k * PY_ROWS);
+            float(*src)[PY_COLS + 1] = buf[k & 1u];
+            for (uint32_t r = 0; r < nr; r++) {
+                acc = acc + src[r][tid];
+            }
+        }
+        __syncthreads();
     }
-    proj[(size_t)b * n + y] = acc;
+    if (tid < ncol) {
+        proj[(size_t)b * n + c0 + tid] = acc;
+    }
 }
 
 // integrate (src/PS/PhaseSpace.cpp:200-239), FP64 accumulation
@@ -199,8 +290,21 @@ __global__ void variance_kernel(uint32_t n, const float* proj,
 int launch_project_x(vfps_mesh* m, const float* src, float* proj)
 {
     const uint32_t rows = m->nb * m->n;
-    hipLaunchKernelGGL(project_x_kernel, dim3(ceil_div(rows, 64)), dim3(64), 0,
-                       m->dev->stream, rows, m->n, src, m->ws, proj);
+    // as many rows per workgroup as the LDS holds, up to PX_ROWS
+    const size_t row_bytes = sizeof(float) * (m->n + 4);
+    const uint32_t rpw = (uint32_t)std::max<size_t>(
+        1, std::min<size_t>(PX_ROWS, m->dev->lds_max / row_bytes));
+    const size_t lds = row_bytes * rpw;
+    VFPS_CHECK_ARG(lds <= m->dev->lds_max,
+                   "updateXProjection: a mesh row exceeds the LDS");
+    if (lds > 65536) {
+        VFPS_HIP(hipFuncSetAttribute(
+            reinterpret_cast<const void*>(&project_x_kernel),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    hipLaunchKernelGGL(project_x_kernel, dim3(ceil_div(rows, rpw)),
+                       dim3(PX_THREADS), lds, m->dev->stream, rows, m->n, rpw,
+                       src, m->ws, proj);
     VFPS_HIP(hipGetLastError());
     return VFPS_OK;
 }
@@ -466,9 +570,9 @@ int vfps_mesh_project_x(vfps_mesh* m)
 int vfps_mesh_project_y(vfps_mesh* m)
 {
     VFPS_CHECK_ARG(m, "vfps_mesh_project_y: NULL mesh");
-    hipLaunchKernelGGL(project_y_kernel, dim3(ceil_div(m->n, 256), m->nb),
-                       dim3(256), 0, m->dev->stream, m->nb, m->n, m->data,
-                       m->ws, m->proj + (size_t)m->nb * m->n);
+    hipLaunchKernelGGL(project_y_kernel, dim3(ceil_div(m->n, PY_COLS), m->nb),
+                       dim3(PY_THREADS), 0, m->dev->stream, m->nb, m->n,
+                       m->data, m->ws, m->proj + (size_t)m->nb * m->n);
     VFPS_HIP(hipGetLastError());
     return VFPS_OK;
 }
