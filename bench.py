@@ -166,6 +166,9 @@ def batched_throughput(steps, log, scheme="kick_drift", tiles=None,
             "cells_per_step": int(cells),
             "steps": steps, "ms_per_step": dt * 1e3 / steps,
             "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
+            "traffic_per_step": (load_traffic("batch_ragged_64",
+                                              "bytes_per_step")
+                                 if scheme == "kick_drift" else None),
             "launch": launch_label(steps, no_graph),
             "tiles_chosen": {str(k): v for k, v in chosen.items()}}
 
@@ -214,6 +217,7 @@ def production_throughput(steps, log, tiles=None, no_graph=False):
             "instances": 16, "n": 1024, "cells_per_step": int(cells),
             "steps": steps, "ms_per_step": dt * 1e3 / steps,
             "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
+            "traffic": load_traffic("batch_production_16x1024"),
             "launch": launch_label(steps, no_graph),
             "tiles_chosen": chosen[0] if chosen else None}
 
@@ -243,11 +247,13 @@ def copy_calibration(mib):
                       "bytes_written": mib * 1048576}), flush=True)
 
 
-def load_traffic(name):
+def load_traffic(name, key="bytes_per_launch"):
+    """HBM bytes from the counter summary profiles/traffic_<name>.json
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)."""
     p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
+            return json.load(f).get(key)
     return None
 
 

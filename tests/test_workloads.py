@@ -57,3 +57,32 @@ def test_scan_setup_detector_threshold_per_scheme():
     assert ScanSetup().tol == 1e-3
     assert ScanSetup(scheme="rotation").tol == 5e-3
     assert ScanSetup(tol=2e-3).tol == 2e-3
+
+
+def test_oracle_renormalize_scales_the_step_by_the_integral(orc):
+    """RenormalizeCharge (src/main.cpp:953-955): at a renormalising step the
+    mesh is divided by its integral after the wake is taken from the
+    unscaled profile and before the kicks.  Every map after the wake is
+    linear in the mesh, so one such step from c * mesh equals the plain step
+    divided by the integral, and the CSR power (from the unscaled profile)
+    is unchanged."""
+    from inovesa_amd.workloads import initial_mesh, kick_drift_config
+    n = 64
+    desc, d = kick_drift_config(n=n)
+    z = orc.z_freespace(d["padded_bins"], np.float32(d["f0"]),
+                        np.float32(d["fmax"]))
+    init = 3.0 * initial_mesh(n, seed=2)
+    runs = {}
+    for every in (0, 1):
+        sim = orc.sim(oracle_cfg(dict(desc, renormalize=every), len(z)),
+                      init, z, z)
+        csr, _ = sim.run(1, want_csr=True)
+        runs[every] = (sim.get()[0].astype(np.float64), csr[0])
+    ws = orc.simpson_weights(n, orc.ruler(n, desc["qmin"], desc["qmax"])[1])
+    fill, _ = orc.integrate(orc.project_x(init.reshape(1, n, n), ws), ws)
+    plain, renorm = runs[0][0], runs[1][0]
+    np.testing.assert_allclose(renorm, plain / float(fill[0]),
+                               rtol=2e-5, atol=2e-6 * np.abs(plain).max()
+                               / float(fill[0]))
+    assert runs[0][1] == runs[1][1]
+    assert abs(float(fill[0]) - 1.0) > 0.5   # the scale is not trivial
