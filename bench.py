@@ -46,14 +46,15 @@ HBM_PEAK_GBS = 8000.0
 
 
 # tilings the autotuner chose on the MI355X for the batch lines (mesh size:
-# rows per tile, workgroup size, persistent), used by --only so that a
-# profiler run times the same kernels without the tuner's launches
+# rows per tile, workgroup size, persistent, tiles per strip), used by
+# --only so that a profiler run times the same kernels without the tuner's
+# launches
 PROFILE_TILES = {
-    "batched": {128: (2, 256, False), 256: (4, 256, False),
-                384: (8, 512, False), 512: (8, 512, False)},
-    "batched_rotation": {128: (2, 256, False), 256: (4, 256, False),
-                         384: (8, 512, False), 512: (8, 512, False)},
-    "production": (4, 512, False),
+    "batched": {128: (2, 256, False, 0), 256: (4, 256, False, 0),
+                384: (8, 512, False, 0), 512: (8, 512, False, 0)},
+    "batched_rotation": {128: (2, 256, False, 0), 256: (4, 256, False, 0),
+                         384: (8, 512, False, 0), 512: (8, 512, False, 0)},
+    "production": (4, 512, False, 8),
 }
 
 
@@ -122,6 +123,14 @@ def launch_label(steps, no_graph):
             "then one finalize launch")
 
 
+def form_label(r):
+    """Launch form of an autotuner row: per-tile, persistent, or strips of
+    k tiles."""
+    if r.get("strip"):
+        return f"strips of {r['strip']} tiles"
+    return "persistent" if r["persistent"] else "per-tile"
+
+
 def batched_throughput(steps, log, scheme="kick_drift", tiles=None,
                        no_graph=False):
     """64 independent meshes of uneven sizes (16 each of 128^2, 256^2,
@@ -139,11 +148,11 @@ def batched_throughput(steps, log, scheme="kick_drift", tiles=None,
     b = Batch([dict(c[0], no_graph=int(no_graph)) for c in cfg],
               [c[2] for c in cfg], [z[0] for z in zs], [z[1] for z in zs])
     if tiles:
-        for size, (T, thr, persistent) in tiles.items():
-            b.retile_size(size, T, thr, persistent=persistent)
+        for size, (T, thr, persistent, strip) in tiles.items():
+            b.retile_size(size, T, thr, persistent=persistent, strip=strip)
         table = [dict(n=size, T=T, threads=thr, persistent=persistent,
-                      us_per_step=0.0, chosen=True)
-                 for size, (T, thr, persistent) in tiles.items()]
+                      strip=strip, us_per_step=0.0, chosen=True)
+                 for size, (T, thr, persistent, strip) in tiles.items()]
     else:
         table = b.autotune(steps=10)
     cells = b.info()["cells"]
@@ -155,8 +164,7 @@ def batched_throughput(steps, log, scheme="kick_drift", tiles=None,
     b.run(steps)
     b.dev.sync()
     dt = time.perf_counter() - t0
-    chosen = {r["n"]: (r["T"], r["threads"],
-                       "persistent" if r["persistent"] else "per-tile",
+    chosen = {r["n"]: (r["T"], r["threads"], form_label(r),
                        round(r["us_per_step"], 2))
               for r in table if r["chosen"]}
     log(f"batched ({scheme}): {len(cfg)} instances, {cells} cells, {steps} "
@@ -194,10 +202,10 @@ def production_throughput(steps, log, tiles=None, no_graph=False):
         zcs.append(zc)
     b = Batch(descs, inits, zws, zcs)
     if tiles:
-        T, thr, persistent = tiles
-        b.retile_size(1024, T, thr, persistent=persistent)
+        T, thr, persistent, strip = tiles
+        b.retile_size(1024, T, thr, persistent=persistent, strip=strip)
         table = [dict(n=1024, T=T, threads=thr, persistent=persistent,
-                      us_per_step=0.0, chosen=True)]
+                      strip=strip, us_per_step=0.0, chosen=True)]
     else:
         table = b.autotune(steps=6)
     cells = b.info()["cells"]
@@ -209,8 +217,7 @@ def production_throughput(steps, log, tiles=None, no_graph=False):
     b.run(steps)
     b.dev.sync()
     dt = time.perf_counter() - t0
-    chosen = [(r["T"], r["threads"],
-               "persistent" if r["persistent"] else "per-tile",
+    chosen = [(r["T"], r["threads"], form_label(r),
                round(r["us_per_step"], 2)) for r in table if r["chosen"]]
     log(f"production: 16 x 1024^2, {steps} steps in {dt:.2f} s")
     return {"value": cells * steps / dt, "unit": "cell-updates/s",

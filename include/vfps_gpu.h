@@ -528,7 +528,8 @@ VFPS_API int vfps_batch_info(vfps_batch* b, uint32_t* ninst,
  * branches.  Results do not depend on it. */
 VFPS_API int vfps_batch_set_merge(vfps_batch* b, int merge);
 /* vfps_batch_retile for every instance of mesh size n at once (one
- * rebuild of the launches); on failure the previous tiling stays */
+ * rebuild of the launches, each in the form of a workgroup per tile or
+ * persistent); on failure the previous tiling stays */
 VFPS_API int vfps_batch_retile_size(vfps_batch* b, uint32_t n,
                                     uint32_t tile_rows, uint32_t threads,
                                     int persistent);
@@ -543,22 +544,37 @@ VFPS_API int vfps_batch_launches(vfps_batch* b, uint32_t* out, uint32_t cap,
  * previous tiling stays. */
 VFPS_API int vfps_batch_retile(vfps_batch* b, uint32_t inst, uint32_t tile_rows,
                                uint32_t threads, int persistent);
-/* vfps_step_config of one instance (out[6], if nout >= 7: persistent) */
+/* The strip form for one instance (kick-drift maps with the direct wake):
+ * a workgroup advances strip_tiles consecutive tiles of tile_rows rows
+ * (at most 8 rows a tile, 64 a strip), kicking each row once and summing
+ * the strip's x projection with one lane per row.  Results do not depend
+ * on it.  On failure the previous tiling stays.  (No reference
+ * counterpart: a launch shape of the batch.) */
+VFPS_API int vfps_batch_retile_strip(vfps_batch* b, uint32_t inst,
+                                     uint32_t tile_rows, uint32_t threads,
+                                     uint32_t strip_tiles);
+/* vfps_batch_retile_strip for every instance of mesh size n at once */
+VFPS_API int vfps_batch_retile_size_strip(vfps_batch* b, uint32_t n,
+                                          uint32_t tile_rows, uint32_t threads,
+                                          uint32_t strip_tiles);
+/* vfps_step_config of one instance (out[6], if nout >= 7: persistent;
+ * out[7], if nout >= 8: tiles per strip, 0 outside the strip form) */
 VFPS_API int vfps_batch_config(vfps_batch* b, uint32_t inst, uint32_t* out,
                                uint32_t nout);
 /* average time of one batch step (all launches), HIP events on the device
  * stream; advances the state by nsteps */
 VFPS_API int vfps_batch_timing(vfps_batch* b, uint32_t nsteps, float* ms_step);
 /* Autotuner: per mesh size, time every tile height T in {1,2,4,8,16},
- * workgroup size in {128,256,512} and launch form (one workgroup per tile,
- * or persistent: as many workgroups as stay resident, each walking several
- * tiles) that fits (steps steps each, on that size's instances alone),
- * time the three fastest again over 4 x steps and keep the fastest of
- * those.  The instances' state is restored afterwards; results do not
- * depend on the choice.  table (cap rows of VFPS_TUNE_COLS floats: n, T,
- * threads, us/step (of the longer timing where re-timed), chosen,
- * persistent, re-timed) may be NULL. */
-#define VFPS_TUNE_COLS 7
+ * workgroup size in {128,256,512} and launch form (one workgroup per tile;
+ * persistent: as many workgroups as stay resident, each walking several
+ * tiles; for the kick-drift maps also strips of 16, 32 or 64 rows in
+ * tiles of 2, 4 or 8) that fits (steps steps each, on that size's
+ * instances alone), time the three fastest again over 4 x steps and keep
+ * the fastest of those.  The instances' state is restored afterwards;
+ * results do not depend on the choice.  table (cap rows of VFPS_TUNE_COLS
+ * floats: n, T, threads, us/step (of the longer timing where re-timed),
+ * chosen, persistent, re-timed, tiles per strip) may be NULL. */
+#define VFPS_TUNE_COLS 8
 VFPS_API int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                                  uint32_t cap, uint32_t* nrows);
 

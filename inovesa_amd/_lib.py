@@ -194,6 +194,8 @@ _SIGS = {
     "vfps_batch_retile": (_int, [_vp, _u32, _u32, _u32, _i32]),
     "vfps_batch_launches": (_int, [_vp, _vp, _u32, _vp]),
     "vfps_batch_retile_size": (_int, [_vp, _u32, _u32, _u32, _i32]),
+    "vfps_batch_retile_strip": (_int, [_vp, _u32, _u32, _u32, _u32]),
+    "vfps_batch_retile_size_strip": (_int, [_vp, _u32, _u32, _u32, _u32]),
     "vfps_batch_set_merge": (_int, [_vp, _i32]),
     "vfps_batch_timing": (_int, [_vp, _u32, _vp]),
     "vfps_batch_autotune": (_int, [_vp, _u32, _vp, _u32, _vp]),

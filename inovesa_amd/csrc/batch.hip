@@ -28,6 +28,8 @@ struct Group {
     uint32_t lds_class = 0;         // LDS bytes of a tile, in 8 KiB units
     uint32_t maxn = 0;              // largest mesh of the group
     bool persist = false;           // grid sized to the resident workgroups
+    uint32_t strip = 0;             // strip form (each member's
+                                    // descriptor holds its tiles per strip)
     uint32_t grid = 0;              // workgroups (0: one per map entry)
     std::vector<uint32_t> members;  // batch indices
     InstDesc* descs_d = nullptr;
@@ -103,7 +105,8 @@ int build_group(vfps_batch* b, Group& g)
     for (uint32_t k : order) {
         const vfps_step* s = b->members[g.members[k]];
         g.maxn = std::max(g.maxn, s->desc.n);
-        for (uint32_t t = 0; t < s->tiles(); t++) {
+        const uint32_t units = g.strip ? s->strips() : s->tiles();
+        for (uint32_t t = 0; t < units; t++) {
             map.push_back((k << 16) | t);
         }
         g.lds = std::max(g.lds, step_lds_of(s));
@@ -126,9 +129,10 @@ int build_group(vfps_batch* b, Group& g)
     VFPS_HIP(hipMalloc(&g.descs_d, sizeof(InstDesc) * descs.size()));
     VFPS_HIP(hipMemcpy(g.descs_d, descs.data(), sizeof(InstDesc) * descs.size(),
                        hipMemcpyHostToDevice));
-    int rc = step_set_lds_limit(g.variant, g.lds);
+    int rc = g.strip ? strip_set_lds_limit(g.variant, g.lds)
+                     : step_set_lds_limit(g.variant, g.lds);
     if (rc != VFPS_OK) return rc;
-    if (g.persist) {
+    if (g.persist && !g.strip) {
         // as many workgroups as stay resident on the device at once, a
         // multiple of 8 (one share per XCD), each walking several tiles
         int per_cu = 0;
@@ -158,7 +162,8 @@ int build_groups(vfps_batch* b)
         Group* g = nullptr;
         for (auto& e : b->groups) {
             if (same_variant(e.variant, s->variant) && e.threads == s->threads &&
-                (b->merge || e.lds_class == cls) && e.persist == s->persist) {
+                (b->merge || e.lds_class == cls) && e.persist == s->persist &&
+                (e.strip != 0) == (s->strip != 0)) {
                 g = &e;
                 break;
             }
@@ -170,6 +175,7 @@ int build_groups(vfps_batch* b)
             g->threads = s->threads;
             g->lds_class = cls;
             g->persist = s->persist;
+            g->strip = s->strip;
         }
         g->members.push_back(i);
     }
@@ -197,9 +203,13 @@ int launch_step(vfps_batch* b, uint32_t parity)
         if (k > 0) {
             VFPS_HIP(hipStreamWaitEvent(st, b->fork, 0));
         }
-        int rc = launch_step_kernel(g.variant, g.descs_d, g.blockmap_d,
-                                    g.nblocks, g.lds, g.threads, g.maxn,
-                                    parity, st, nullptr, g.grid);
+        int rc = g.strip
+                     ? launch_strip_kernel(g.variant, g.descs_d, g.blockmap_d,
+                                           g.nblocks, g.lds, g.threads, g.maxn,
+                                           parity, st)
+                     : launch_step_kernel(g.variant, g.descs_d, g.blockmap_d,
+                                          g.nblocks, g.lds, g.threads, g.maxn,
+                                          parity, st, nullptr, g.grid);
         if (rc != VFPS_OK) return rc;
         if (k > 0) {
             VFPS_HIP(hipEventRecord(g.done, st));
@@ -257,6 +267,46 @@ int batch_graph(vfps_batch* b, uint32_t steps, uint32_t p, hipGraphExec_t* out)
 void sync_parity(vfps_batch* b)
 {
     for (auto* s : b->members) s->parity = b->parity;
+}
+
+// Retile the members `pick` selects (persistent: a resident grid walking
+// the tiles; strip: tiles per strip, 0 for a workgroup per tile).  On
+// failure every member keeps its previous tiling.
+template <typename Pick>
+int retile_members(vfps_batch* b, Pick pick, uint32_t T, uint32_t threads,
+                   bool persistent, uint32_t strip)
+{
+    VFPS_HIP(hipStreamSynchronize(b->dev->stream));
+    sync_parity(b);
+    struct Old {
+        vfps_step* s;
+        uint32_t T, thr, strip;
+        bool p;
+    };
+    std::vector<Old> old;
+    int rc = VFPS_OK;
+    for (uint32_t i = 0; i < b->members.size(); i++) {
+        vfps_step* s = b->members[i];
+        if (!pick(i, s)) continue;
+        old.push_back({s, s->T, s->threads, s->strip, s->persist});
+        s->persist = persistent && strip == 0;
+        s->strip = strip;
+        if ((rc = step_retile(s, T, threads)) != VFPS_OK) break;
+    }
+    if (rc == VFPS_OK) {
+        rc = build_groups(b);
+    }
+    if (rc != VFPS_OK) {
+        const std::string msg = vfps_last_error();
+        for (auto& o : old) {
+            o.s->persist = o.p;
+            o.s->strip = o.strip;
+            step_retile(o.s, o.T, o.thr);
+        }
+        build_groups(b);
+        return fail(rc, msg.c_str());
+    }
+    return VFPS_OK;
 }
 
 void batch_free(vfps_batch* b)
@@ -495,24 +545,25 @@ int vfps_batch_retile(vfps_batch* b, uint32_t inst, uint32_t tile_rows,
                        threads <= STEP_MAX_THREADS,
                    "vfps_batch_retile: tile rows or workgroup size");
     VFPS_TRY_BEGIN
-    VFPS_HIP(hipStreamSynchronize(b->dev->stream));
-    sync_parity(b);
-    vfps_step* s = b->members[inst];
-    const uint32_t T0 = s->T, thr0 = s->threads;
-    const bool p0 = s->persist;
-    s->persist = persistent != 0;
-    int rc = step_retile(s, tile_rows, threads);
-    if (rc == VFPS_OK) {
-        rc = build_groups(b);
-    }
-    if (rc != VFPS_OK) {
-        // keep the batch runnable with the previous tiling
-        s->persist = p0;
-        step_retile(s, T0, thr0);
-        build_groups(b);
-        return rc;
-    }
-    return VFPS_OK;
+    return retile_members(
+        b, [&](uint32_t i, const vfps_step*) { return i == inst; }, tile_rows,
+        threads, persistent != 0, 0);
+    VFPS_TRY_END
+}
+
+int vfps_batch_retile_strip(vfps_batch* b, uint32_t inst, uint32_t tile_rows,
+                            uint32_t threads, uint32_t strip_tiles)
+{
+    VFPS_CHECK_ARG(b && inst < b->members.size(), "vfps_batch_retile_strip: "
+                                                  "bad instance");
+    VFPS_CHECK_ARG(tile_rows >= 1 && threads >= 64 && threads % 64 == 0 &&
+                       threads <= STEP_MAX_THREADS && strip_tiles >= 1,
+                   "vfps_batch_retile_strip: tile rows, workgroup size or "
+                   "tiles per strip");
+    VFPS_TRY_BEGIN
+    return retile_members(
+        b, [&](uint32_t i, const vfps_step*) { return i == inst; }, tile_rows,
+        threads, false, strip_tiles);
     VFPS_TRY_END
 }
 
@@ -535,34 +586,24 @@ int vfps_batch_retile_size(vfps_batch* b, uint32_t n, uint32_t tile_rows,
                        threads <= STEP_MAX_THREADS,
                    "vfps_batch_retile_size: tile rows or workgroup size");
     VFPS_TRY_BEGIN
-    VFPS_HIP(hipStreamSynchronize(b->dev->stream));
-    sync_parity(b);
-    struct Old {
-        vfps_step* s;
-        uint32_t T, thr;
-        bool p;
-    };
-    std::vector<Old> old;
-    int rc = VFPS_OK;
-    for (auto* s : b->members) {
-        if (s->desc.n != n) continue;
-        old.push_back({s, s->T, s->threads, s->persist});
-        s->persist = persistent != 0;
-        if ((rc = step_retile(s, tile_rows, threads)) != VFPS_OK) break;
-    }
-    if (rc == VFPS_OK) {
-        rc = build_groups(b);
-    }
-    if (rc != VFPS_OK) {
-        // keep the batch runnable with the previous tiling
-        for (auto& o : old) {
-            o.s->persist = o.p;
-            step_retile(o.s, o.T, o.thr);
-        }
-        build_groups(b);
-        return rc;
-    }
-    return VFPS_OK;
+    return retile_members(
+        b, [&](uint32_t, const vfps_step* s) { return s->desc.n == n; },
+        tile_rows, threads, persistent != 0, 0);
+    VFPS_TRY_END
+}
+
+int vfps_batch_retile_size_strip(vfps_batch* b, uint32_t n, uint32_t tile_rows,
+                                 uint32_t threads, uint32_t strip_tiles)
+{
+    VFPS_CHECK_ARG(b, "vfps_batch_retile_size_strip: NULL batch");
+    VFPS_CHECK_ARG(tile_rows >= 1 && threads >= 64 && threads % 64 == 0 &&
+                       threads <= STEP_MAX_THREADS && strip_tiles >= 1,
+                   "vfps_batch_retile_size_strip: tile rows, workgroup size "
+                   "or tiles per strip");
+    VFPS_TRY_BEGIN
+    return retile_members(
+        b, [&](uint32_t, const vfps_step* s) { return s->desc.n == n; },
+        tile_rows, threads, false, strip_tiles);
     VFPS_TRY_END
 }
 
@@ -636,11 +677,12 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
     // time one tiling of one size class on its instances alone: us per step
     // (dup: a persistent grid as large as the map, i.e. the plain launch)
     auto time_config = [&](const std::vector<uint32_t>& idx, uint32_t T,
-                           uint32_t thr, bool pers, uint32_t nsteps, float* us,
-                           bool* dup) -> int {
+                           uint32_t thr, bool pers, uint32_t strip,
+                           uint32_t nsteps, float* us, bool* dup) -> int {
         *dup = false;
         for (uint32_t i : idx) {
             b->members[i]->persist = pers;
+            b->members[i]->strip = strip;
             int r0 = step_retile(b->members[i], T, thr);
             if (r0 != VFPS_OK) return r0;
         }
@@ -676,6 +718,7 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
     struct Cand {
         uint32_t T, thr;
         bool pers;
+        uint32_t strip;
         float us;
         uint32_t row;
         bool refined;
@@ -689,23 +732,51 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
     std::vector<Choice> choices;
     for (auto& cl : classes) {
         const uint32_t n = cl.first;
+        struct Orig {
+            uint32_t T, thr, strip;
+            bool pers;
+        };
+        std::vector<Orig> orig;
+        for (uint32_t i : cl.second) {
+            const vfps_step* m = b->members[i];
+            orig.push_back({m->T, m->threads, m->strip, m->persist});
+        }
+        // strips exist for the kick-drift maps with the direct wake
+        const bool strips_ok =
+            strip_kernel_for(b->members[cl.second[0]]->variant, 1) != nullptr;
         std::vector<Cand> cands;
         for (uint32_t thr : {128u, 256u, 512u}) {
+            if (ceil_div(n, thr) > (uint32_t)STEP_MAX_COLS) continue;
+            // (T, persistent, tiles per strip): a workgroup per tile, a
+            // resident grid, strips of 16, 32 or 64 rows
+            std::vector<std::tuple<uint32_t, bool, uint32_t>> forms;
             for (uint32_t T : {1u, 2u, 4u, 8u, 16u}) {
-                for (bool pers : {false, true}) {
-                    if (ceil_div(n, thr) > (uint32_t)STEP_MAX_COLS || T > n) continue;
-                    float us = 0;
-                    bool dup = false;
-                    const int r = time_config(cl.second, T, thr, pers, even, &us, &dup);
-                    if (r == VFPS_ERR_INVALID) continue;  // does not fit
-                    if (r != VFPS_OK) {
-                        rc = r;
-                        break;
+                forms.emplace_back(T, false, 0u);
+                forms.emplace_back(T, true, 0u);
+            }
+            if (strips_ok) {
+                for (uint32_t T : {2u, 4u, 8u}) {
+                    for (uint32_t S : {16u, 32u, 64u}) {
+                        if (S > T && S <= n) forms.emplace_back(T, false, S / T);
                     }
-                    if (dup) continue;
-                    cands.push_back({T, thr, pers, us, rows++, false});
                 }
-                if (rc != VFPS_OK) break;
+            }
+            for (const auto& f : forms) {
+                const uint32_t T = std::get<0>(f);
+                const bool pers = std::get<1>(f);
+                const uint32_t strip = std::get<2>(f);
+                if (T > n) continue;
+                float us = 0;
+                bool dup = false;
+                const int r = time_config(cl.second, T, thr, pers, strip, even,
+                                          &us, &dup);
+                if (r == VFPS_ERR_INVALID) continue;  // does not fit
+                if (r != VFPS_OK) {
+                    rc = r;
+                    break;
+                }
+                if (dup) continue;
+                cands.push_back({T, thr, pers, strip, us, rows++, false});
             }
             if (rc != VFPS_OK) break;
         }
@@ -717,7 +788,7 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
         for (size_t k = 0; k < cands.size() && k < 3 && rc == VFPS_OK; k++) {
             bool dup = false;
             rc = time_config(cl.second, cands[k].T, cands[k].thr, cands[k].pers,
-                             4 * even, &cands[k].us, &dup);
+                             cands[k].strip, 4 * even, &cands[k].us, &dup);
             cands[k].refined = true;
         }
         if (rc != VFPS_OK) break;
@@ -735,13 +806,16 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
             }
             choices.push_back(ch);
         }
-        for (uint32_t i : cl.second) {
-            vfps_step* m = b->members[i];
+        for (size_t j = 0; j < cl.second.size(); j++) {
+            vfps_step* m = b->members[cl.second[j]];
             if (best) {
                 m->persist = best->pers;
+                m->strip = best->strip;
                 rc = step_retile(m, best->T, best->thr);
             } else {
-                rc = step_retile(m, m->T, m->threads);
+                m->persist = orig[j].pers;
+                m->strip = orig[j].strip;
+                rc = step_retile(m, orig[j].T, orig[j].thr);
             }
             if (rc != VFPS_OK) break;
         }
@@ -756,6 +830,7 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                 r[4] = (&c == best) ? 1.0f : 0.0f;
                 r[5] = c.pers ? 1.0f : 0.0f;
                 r[6] = c.refined ? 1.0f : 0.0f;
+                r[7] = (float)c.strip;
             }
         }
         if (rc != VFPS_OK) break;
@@ -795,6 +870,7 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
             if ((rc = time_all(&t_pers)) != VFPS_OK) break;
             for (uint32_t i : classes[ch.n]) {
                 b->members[i]->persist = false;
+                b->members[i]->strip = ch.best_tile.strip;
                 if ((rc = step_retile(b->members[i], ch.best_tile.T,
                                       ch.best_tile.thr)) != VFPS_OK) break;
             }
@@ -804,6 +880,7 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                 // back to the persistent choice
                 for (uint32_t i : classes[ch.n]) {
                     b->members[i]->persist = true;
+                    b->members[i]->strip = 0;
                     if ((rc = step_retile(b->members[i], ch.best.T,
                                           ch.best.thr)) != VFPS_OK) break;
                 }

@@ -28,6 +28,7 @@
 #include "step.hpp"
 #include "step_kernel.hpp"
 #include "step_select.hpp"
+#include "strip_kernel.hpp"
 
 namespace vfps_impl {
 
@@ -74,6 +75,9 @@ size_t step_lds_bytes(const StepVariant& v, uint32_t n, uint32_t T,
 
 size_t step_lds_of(const vfps_step* s)
 {
+    if (s->strip) {
+        return strip_lds_bytes(s->desc.n, s->T, s->strip, s->stride - s->T);
+    }
     return step_lds_bytes(s->variant, s->desc.n, s->T, s->stride, !s->in_batch);
 }
 
@@ -497,6 +501,8 @@ static int step_upload_desc(vfps_step* s)
                    "size");
     d.T = s->T;
     d.stride = s->stride;
+    d.K = s->strip;
+    d.ringw = s->strip ? strip_ring(s->T, hlo + hhi) : 0u;
     s->inst_h = d;
 
     FieldParams p{};
@@ -666,6 +672,21 @@ int step_retile(vfps_step* s, uint32_t T, uint32_t threads)
                    "retile: tile rows must be in [1, min(threads, n)]");
     VFPS_CHECK_ARG(ceil_div(s->desc.n, threads) <= (uint32_t)STEP_MAX_COLS,
                    "retile: too few threads for the mesh size");
+    if (s->strip) {
+        // the strip form: kick-drift maps with the direct wake, in a batch;
+        // one wavefront sums the x projection of a strip, a row per lane
+        VFPS_CHECK_ARG(s->in_batch && strip_kernel_for(s->variant, 1) != nullptr &&
+                           !s->desc.fast_projection,
+                       "retile: strips need a batch instance with the "
+                       "kick-drift maps, the direct wake and the reference "
+                       "order of the x projection");
+        VFPS_CHECK_ARG(T <= STRIP_MAX_T && T * s->strip <= STRIP_MAX_ROWS &&
+                           s->desc.n % 4 == 0 &&
+                           (strip_ring(T, s->hmax) + T) * s->desc.n >=
+                               8 * T * s->strip,
+                       "retile: strips of at most 64 rows in tiles of at most "
+                       "8 rows");
+    }
     const uint32_t oldT = s->T, oldthr = s->threads;
     s->T = T;
     s->threads = threads;
@@ -674,7 +695,10 @@ int step_retile(vfps_step* s, uint32_t T, uint32_t threads)
     if (rc == VFPS_OK && lds > s->dev->lds_max) {
         rc = fail(VFPS_ERR_INVALID, "retile: tile exceeds the LDS of one CU");
     }
-    if (rc == VFPS_OK) rc = step_set_lds_limit(s->variant, lds);
+    if (rc == VFPS_OK) {
+        rc = s->strip ? strip_set_lds_limit(s->variant, lds)
+                      : step_set_lds_limit(s->variant, lds);
+    }
     if (rc != VFPS_OK) {
         s->T = oldT;
         s->threads = oldthr;
@@ -1182,6 +1206,9 @@ int vfps_step_config(vfps_step* s, uint32_t* out, uint32_t nout)
     out[5] = s->fused ? (uint32_t)step_lds_of(s) : 0u;
     if (nout >= 7) {
         out[6] = s->persist ? 1u : 0u;
+    }
+    if (nout >= 8) {
+        out[7] = s->strip;
     }
     return VFPS_OK;
 }

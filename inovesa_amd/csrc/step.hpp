@@ -46,6 +46,8 @@ struct InstDesc {
                                   // closes the step (direct form) or by
                                   // the field kernel
     uint32_t* err;                // set if a tap falls outside the halo
+    uint32_t K;                   // strip form: tiles per strip (0: none)
+    uint32_t ringw;               // strip form: rows of the kicked-row ring
     uint32_t seq_proj;            // 1: x projection in the reference's order
     uint32_t norm_every;          // > 0: renormalise the charge before the
                                   // maps of every step whose count is a
@@ -88,6 +90,18 @@ int launch_step_kernel(const StepVariant& v, const InstDesc* descs_d,
 // workgroups of this launch shape that stay resident on one CU
 int step_occupancy(const StepVariant& v, uint32_t threads, uint32_t maxn,
                    size_t lds_bytes, int* per_cu);
+// Strip form (strip_kernel.hpp; kick-drift maps, direct wake, batches): a
+// workgroup advances K tiles of one instance.
+const void* strip_kernel_for(const StepVariant& v, uint32_t maxc);
+size_t strip_lds_bytes(uint32_t n, uint32_t T, uint32_t K, uint32_t halo);
+uint32_t strip_ring(uint32_t T, uint32_t halo);
+int strip_set_lds_limit(const StepVariant& v, size_t bytes);
+// nblocks (a multiple of 8) workgroups, one per block-map entry
+// (instance << 16 | strip)
+int launch_strip_kernel(const StepVariant& v, const InstDesc* descs_d,
+                        const uint32_t* blockmap_d, uint32_t nblocks,
+                        size_t lds_bytes, uint32_t threads, uint32_t maxn,
+                        uint32_t parity, hipStream_t st);
 // FP64 Toeplitz kernels of the direct wake (g) and CSR power (h) forms
 void toeplitz_tables(uint32_t n, uint32_t M, const float2* Z,
                      const float2* Zc, const float* renorm, float fdelta,
@@ -147,6 +161,8 @@ struct vfps_step {
     vfps_impl::StepVariant variant{};
     uint32_t T = 1, threads = 256, stride = 1, hmax = 0, M = 0;
     bool persist = false;  // in a batch: persistent launch (grid < tiles)
+    uint32_t strip = 0;    // in a batch: tiles per strip (strip form; 0: a
+                           // workgroup per tile)
     bool in_batch = false; // a batch member (the batch forms' LDS layout)
     bool fused = true;
     bool direct = true;
@@ -159,6 +175,7 @@ struct vfps_step {
     std::map<uint32_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 
     uint32_t tiles() const { return vfps_impl::ceil_div(desc.n, T); }
+    uint32_t strips() const { return vfps_impl::ceil_div(desc.n, T * strip); }
 };
 
 namespace vfps_impl {

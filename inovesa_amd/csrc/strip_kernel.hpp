@@ -1,0 +1,654 @@
+// Strip form of the batched kick-drift step (scheme 0, direct wake).
+//
+// A workgroup advances a strip of S = K * T consecutive output rows of one
+// instance, in K sub-tiles of T rows, instead of one tile of T rows:
+//   A  once per strip: the previous x projection, the per-column drift,
+//      Fokker-Planck and Simpson tables, the RF taps of every row the strip
+//      kicks;
+//   B  once per strip: the direct wake offsets of the kicked rows and the
+//      CSR dot products of the output rows, STRIP_DOT_ROWS rows per
+//      wavefront sharing each load of the profile (each row's sum keeps the
+//      lane order and the wave_sum_dpp of the tile kernel, so the bits are
+//      the same), then the tiles' CSR shares;
+//   then per sub-tile:
+//   C, D  the wake and RF kicks of the rows that enter the drift's window,
+//      from the input rows staged in LDS into a ring of kicked rows (a row
+//      is kicked once per strip, not once per tile that reads it);
+//   E  the drift along x from the ring, a sliding window of consecutive
+//      rows per column;
+//   F  Fokker-Planck along y and the store of the output rows;
+//   and at the end:
+//   G  the x projection of the strip's rows, one row per lane of one
+//      wavefront, summed in std::inner_product order from the rows just
+//      stored;
+//   H  the CSR shares were stored in B; strip 0 closes the step.
+// Every output value is the same sum, in the same order, as in the tile
+// kernel, so the two forms give the same bits (pinned by the batch tests).
+#pragma once
+
+#include "step_kernel.hpp"
+
+namespace vfps_impl {
+
+// most output rows of one strip: its x projection is summed by one
+// wavefront, one row per lane
+constexpr uint32_t STRIP_MAX_ROWS = 64;
+// most rows of one sub-tile
+constexpr uint32_t STRIP_MAX_T = 8;
+// rows of the direct-form dot products one wavefront sums together
+constexpr int STRIP_DOT_ROWS = 8;
+// float4 loads of the input rows one thread keeps in flight
+constexpr int STRIP_LOAD_BATCH = 2;
+
+// rows of the ring of kicked rows: a power of two that holds the drift's
+// window of T + halo rows
+__host__ __device__ inline uint32_t strip_ring_rows(uint32_t T, uint32_t halo)
+{
+    uint32_t r = 1;
+    while (r < T + halo) r <<= 1;
+    return r;
+}
+
+// LDS layout of the strip kernel (4-byte words; FP64 arrays first)
+struct StripLds {
+    uint32_t gwin, hwin;   // FP64 windows of the wake and CSR kernels
+    uint32_t wsum, csum;   // FP64 dot products of the wake rows, CSR rows
+    uint32_t pprev;        // previous x projection
+    uint32_t wsl;          // Simpson weights
+    uint32_t tapw, tapr;   // kick taps of every kicked row of the strip
+    uint32_t ring;         // ring of kicked rows, row r at (r + hlo) % rows
+    uint32_t stage;        // [T][n] input rows, then the drift's output
+    uint32_t words;
+};
+
+__host__ __device__ inline StripLds strip_lds(uint32_t n, uint32_t T, uint32_t K,
+                                              uint32_t halo)
+{
+    StripLds L{};
+    const uint32_t S = T * K;
+    const uint32_t nk = S + halo;  // rows one strip kicks, at most
+    uint32_t w = 0;
+    L.gwin = w;
+    w += 2 * (nk + n);
+    L.hwin = w;
+    w += 2 * (S + n);
+    L.wsum = w;
+    w += 2 * nk;
+    L.csum = w;
+    w += 2 * S;
+    w = step_round4(w);
+    L.pprev = w;
+    w += step_round4(n);
+    L.wsl = w;
+    w += step_round4(n);
+    L.tapw = w;
+    w += STEP_TAP_WORDS * nk;
+    L.tapr = w;
+    w += STEP_TAP_WORDS * nk;
+    L.ring = w;
+    w += strip_ring_rows(T, halo) * step_round4(n);
+    L.stage = w;
+    w += T * step_round4(n);
+    L.words = w;
+    return L;
+}
+
+// The wake kick and then the RF kick along y (or the RF kick alone) of the
+// rows [a, a + c) staged in LDS, into their ring slots.  The same sums in
+// the same order as kick2_rows / kick_rows; a group of four outputs whose
+// inputs all lie inside the row skips the per-sample checks.
+template <int IT>
+__device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
+                                           float* __restrict__ ring, uint32_t n,
+                                           int a, int c, int k0, uint32_t hlo,
+                                           uint32_t rmask,
+                                           const float* __restrict__ tapw,
+                                           const float* __restrict__ tapr,
+                                           bool wake)
+{
+    const uint32_t bs = blockDim.x;
+    const uint32_t ng = n >> 2;
+    const uint32_t drow = bs / ng, dg = bs % ng;
+    uint32_t row = threadIdx.x / ng;
+    uint32_t g = threadIdx.x % ng;
+    const int in = (int)n;
+    while (row < (uint32_t)c) {
+        const int r = a + (int)row;
+        const float* t2 = tapr + (uint32_t)(r - k0) * STEP_TAP_WORDS;
+        const vec4f w2 = *reinterpret_cast<const vec4f*>(t2);
+        const int32_t base2 = __float_as_int(t2[4]);
+        const float* s = stage + (size_t)row * n;
+        const int32_t y0 = (int32_t)(4 * g);
+        vec4f o;
+        if (wake) {
+            const float* t1 = tapw + (uint32_t)(r - k0) * STEP_TAP_WORDS;
+            const vec4f w1 = *reinterpret_cast<const vec4f*>(t1);
+            const int32_t base1 = __float_as_int(t1[4]);
+            const int32_t clo = y0 + base2;
+            const int32_t lo = clo + base1;
+            const bool inside = lo >= 0 && lo + 2 * IT + 1 < in && clo >= 0 &&
+                                clo + IT + 2 < in;
+            float v[2 * IT + 2];
+            if (inside) {
+#pragma unroll
+                for (int k = 0; k < 2 * IT + 2; k++) {
+                    v[k] = s[lo + k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 2 * IT + 2; k++) {
+                    const uint32_t ys = (uint32_t)(lo + k);
+                    v[k] = ys < n ? s[ys] : 0.0f;
+                }
+            }
+            float cc[IT + 3];
+#pragma unroll
+            for (int k = 0; k < IT + 3; k++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int j = 0; j < IT; j++) {
+                    acc = acc + v[k + j] * w1[j];
+                }
+                cc[k] = (inside || (uint32_t)(clo + k) < n) ? acc : 0.0f;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int j = 0; j < IT; j++) {
+                    acc = acc + cc[e + j] * w2[j];
+                }
+                o[e] = acc;
+            }
+        } else {
+            const int32_t lo = y0 + base2;
+            const bool inside = lo >= 0 && lo + IT + 2 < in;
+            float v[IT + 3];
+            if (inside) {
+#pragma unroll
+                for (int k = 0; k < IT + 3; k++) {
+                    v[k] = s[lo + k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < IT + 3; k++) {
+                    const uint32_t ys = (uint32_t)(lo + k);
+                    v[k] = ys < n ? s[ys] : 0.0f;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int j = 0; j < IT; j++) {
+                    acc = acc + v[e + j] * w2[j];
+                }
+                o[e] = acc;
+            }
+        }
+        const uint32_t slot = ((uint32_t)r + hlo) & rmask;
+        *reinterpret_cast<vec4f*>(ring + (size_t)slot * n + 4 * g) = o;
+        g += dg;
+        row += drow;
+        if (g >= ng) {
+            g -= ng;
+            row++;
+        }
+    }
+}
+
+template <int IT, int IPFP, int MAXC>
+__device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
+                                           uint32_t parity, float* lds,
+                                           StepShared& sh)
+{
+    const uint32_t n = d.n;
+    const uint32_t T = d.T;
+    const uint32_t K = d.K;
+    const uint32_t S = T * K;
+    const int in = (int)n;
+    const int s0 = (int)(strip * S);
+    if (s0 >= in) {
+        return;  // not launched: maps hold exactly the strips
+    }
+    const int srows = min((int)S, in - s0);
+    const int hlo = (int)d.hlo, hhi = (int)d.hhi;
+    const uint32_t rmask = d.ringw - 1u;
+    // the rows the strip kicks: every row a drift window of it reads
+    const int k0 = max(0, s0 - hlo);
+    const int k1 = min(in, s0 + srows + hhi);
+    const int nk = k1 - k0;
+    const StripLds L = strip_lds(n, T, K, d.hlo + d.hhi);
+    double* gwin = reinterpret_cast<double*>(lds + L.gwin);
+    double* hwin = reinterpret_cast<double*>(lds + L.hwin);
+    double* wsum = reinterpret_cast<double*>(lds + L.wsum);
+    double* csum = reinterpret_cast<double*>(lds + L.csum);
+    float* pprev = lds + L.pprev;
+    float* wsl = lds + L.wsl;
+    float* tapw = lds + L.tapw;
+    float* tapr = lds + L.tapr;
+    float* ring = lds + L.ring;
+    float* stage = lds + L.stage;
+    const gfloat* in_mesh = (const gfloat*)d.mesh[parity];
+    gfloat* out = (gfloat*)d.mesh[parity ^ 1u];
+    const gfloat* proj_in = (const gfloat*)d.proj[parity];
+    gfloat* proj_out = (gfloat*)d.proj[parity ^ 1u];
+    gfloat* prof_hist = (gfloat*)d.prof_hist;
+    const gfloat* ws_g = (const gfloat*)d.ws;
+    const gfloat* rf_off = (const gfloat*)d.rf_off;
+    const gfloat* drift_off = (const gfloat*)d.drift_off;
+    const gfloat* fp_w = (const gfloat*)d.fp_w;
+    const guint* fp_i = (const guint*)d.fp_i;
+    const gdouble* wake_g = (const gdouble*)d.wake_g;
+    const gdouble* csr_h = (const gdouble*)d.csr_h;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t lane = tid & 63u, wave = tid >> 6, nw = bs >> 6;
+    const int32_t half = (int32_t)(n / 2);
+    const bool wake = d.use_wake != 0;
+    const bool csr = d.use_csr != 0;
+
+    // ---- A: the strip's tables ---------------------------------------------
+    uint32_t cnt_v = 0;
+    if (tid == 0 && d.counter) {
+        cnt_v = d.counter[parity];
+    }
+    for (uint32_t x = tid; x < n; x += bs) {
+        pprev[x] = proj_in[x];
+        wsl[x] = ws_g[x];
+    }
+    // the kernel windows the strip's dot products read: row r of the wake
+    // needs g[r + n - 1 - x'] (r in [k0, k1)), CSR row r h[r + n - 1 - x']
+    // (r in [s0, s0 + srows))
+    if (wake) {
+        const uint32_t gl = (uint32_t)nk + n - 1u;
+        for (uint32_t k = tid; k < gl; k += bs) gwin[k] = wake_g[(uint32_t)k0 + k];
+    }
+    if (csr) {
+        const uint32_t hl = (uint32_t)srows + n - 1u;
+        for (uint32_t k = tid; k < hl; k += bs) hwin[k] = csr_h[(uint32_t)s0 + k];
+    }
+    // per-column tables: the drift taps' first row and weights (the taps
+    // are consecutive unless one falls outside the mesh), the
+    // Fokker-Planck stencil's first index and weights (consecutive away from
+    // the mesh edges and the row-floor(ycenter) quirk); columns whose taps
+    // are not consecutive reread their tables in the sub-tiles
+    const int ncols = (int)((n + bs - 1) / bs);
+    uint32_t col_d0[MAXC];
+    float col_dw[MAXC][IT];
+    uint32_t col_f0[MAXC];
+    float col_fw[MAXC][IPFP > 0 ? IPFP : 1];
+    uint32_t dcons = 0, fcons = 0;  // bit c: column c's taps are consecutive
+#pragma unroll
+    for (int c = 0; c < MAXC; c++) {
+        const uint32_t y = tid + (uint32_t)c * bs;
+        col_d0[c] = 0;
+        col_f0[c] = 0;
+        if (c < ncols && y < n) {
+            uint32_t idx[IT];
+            kick_taps<IT>(drift_off[y], n, idx, col_dw[c]);
+            bool cons = true;
+#pragma unroll
+            for (int j = 1; j < IT; j++) {
+                cons = cons && idx[j] == idx[0] + (uint32_t)j;
+            }
+            dcons |= cons ? (1u << c) : 0u;
+            col_d0[c] = idx[0];
+            bool fc = true;
+#pragma unroll
+            for (int k = 0; k < IPFP; k++) {
+                const uint32_t fi = fp_i[(size_t)k * n + y];
+                col_fw[c][k] = fp_w[(size_t)k * n + y];
+                if (k == 0) col_f0[c] = fi;
+                fc = fc && fi == col_f0[c] + (uint32_t)k;
+            }
+            fcons |= fc ? (1u << c) : 0u;
+        }
+    }
+    for (int i = (int)tid; i < nk; i += (int)bs) {
+        store_row_taps<IT>(rf_off[k0 + i], n, tapr + (uint32_t)i * STEP_TAP_WORDS);
+    }
+    if (tid == 0) {
+        sh.cnt = cnt_v;
+    }
+    __syncthreads();
+    const uint32_t cnt = sh.cnt;
+    // PhaseSpace::integrateAndNormalize on the steps the reference
+    // renormalises, as in the tile kernel (the same FP64 sum, so every
+    // strip and every tile finds the same factor)
+    const bool renorm = d.norm_every != 0 && cnt % d.norm_every == 0;
+    float norm = 1.0f;
+    if (renorm) {
+        if (wave == 0) {
+            double f = 0.0;
+            for (uint32_t x = lane; x < n; x += 64) {
+                f += (double)pprev[x] * (double)ws_g[x];
+            }
+            f = wave_sum_dpp(f);
+            if (lane == 0) {
+                sh.norm = d.fill_set > 0.0f ? d.fill_set / (float)f : 0.0f;
+            }
+        }
+        __syncthreads();
+        norm = sh.norm;
+    }
+
+    // ---- B: wake offsets of the kicked rows, CSR rows of the output rows ----
+    // A job is up to STRIP_DOT_ROWS consecutive rows of one table: lane l
+    // takes x' = l, l + 64, ... and row r accumulates g[r + n - 1 - x'] p[x']
+    // (the tile kernel's window order), then wave_sum_dpp.
+    {
+        const int nwj = wake ? (nk + STRIP_DOT_ROWS - 1) / STRIP_DOT_ROWS : 0;
+        const int ncj = csr ? (srows + STRIP_DOT_ROWS - 1) / STRIP_DOT_ROWS : 0;
+        for (int job = (int)wave; job < nwj + ncj; job += (int)nw) {
+            const bool isw = job < nwj;
+            const int r_first = isw ? k0 + STRIP_DOT_ROWS * job
+                                    : s0 + STRIP_DOT_ROWS * (job - nwj);
+            const int rows = isw ? min(STRIP_DOT_ROWS, k1 - r_first)
+                                 : min(STRIP_DOT_ROWS, s0 + srows - r_first);
+            const double* tbl =
+                isw ? gwin + (r_first - k0 + in - 1) : hwin + (r_first - s0 + in - 1);
+            double acc[STRIP_DOT_ROWS];
+#pragma unroll
+            for (int i = 0; i < STRIP_DOT_ROWS; i++) {
+                acc[i] = 0.0;
+            }
+            if (rows == STRIP_DOT_ROWS) {
+                for (uint32_t x = lane; x < n; x += 64) {
+                    const double p = (double)pprev[x];
+                    const double* t = tbl - x;
+#pragma unroll
+                    for (int i = 0; i < STRIP_DOT_ROWS; i++) {
+                        acc[i] += t[i] * p;
+                    }
+                }
+            } else {
+                for (uint32_t x = lane; x < n; x += 64) {
+                    const double p = (double)pprev[x];
+                    const double* t = tbl - x;
+#pragma unroll
+                    for (int i = 0; i < STRIP_DOT_ROWS; i++) {
+                        if (i < rows) acc[i] += t[i] * p;
+                    }
+                }
+            }
+            double* dst = isw ? wsum + (r_first - k0) : csum + (r_first - s0);
+#pragma unroll
+            for (int i = 0; i < STRIP_DOT_ROWS; i++) {
+                if (i < rows) {
+                    const double v = wave_sum_dpp(acc[i]);
+                    if (lane == 0) dst[i] = v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // wake taps of the kicked rows, the wake offsets of the output rows
+    if (wake) {
+        for (int i = (int)tid; i < nk; i += (int)bs) {
+            const float off = d.wake_scale * (float)wsum[i];
+            store_row_taps<IT>(off, n, tapw + (uint32_t)i * STEP_TAP_WORDS);
+            const int r = k0 + i;
+            if (d.wake_out && r >= s0 && r < s0 + srows) {
+                d.wake_out[r] = off;
+            }
+        }
+    }
+    // each tile's CSR share: sum over its rows of p[x] * (H p)[x], in order
+    if (csr) {
+        for (uint32_t k = tid; k < K; k += bs) {
+            const int x0 = s0 + (int)(k * T);
+            if (x0 >= in) continue;
+            const int rows_out = min((int)T, in - x0);
+            double q = 0.0;
+            for (int o = 0; o < rows_out; o++) {
+                q += (double)pprev[x0 + o] * csum[x0 + o - s0];
+            }
+            d.csr_part[(size_t)parity * n + strip * K + k] = q;
+        }
+    }
+    __syncthreads();
+
+    // ---- sub-tiles ----------------------------------------------------------
+    const uint32_t n4 = n >> 2;
+    for (uint32_t k = 0; k < K; k++) {
+        const int x0 = s0 + (int)(k * T);
+        if (x0 >= in) break;
+        const int rows_out = min((int)T, in - x0);
+        // the drift's window of kicked rows, and the rows entering it
+        const int wlo = x0 - hlo, whi = x0 + rows_out + hhi;
+        const int na = k == 0 ? k0 : min(k1, x0 + hhi);
+        const int nb = min(k1, whi);
+        // rows of the window outside the mesh read as zeros (they enter
+        // the window once: below the mesh in the first sub-tile, above it
+        // as the window passes the last row)
+        const int zlo = k == 0 ? wlo : max(x0 + hhi, in);
+        bool zeroed = false;
+        for (int r = zlo; r < whi; r++) {
+            if (r >= 0 && r < in) continue;
+            zeroed = true;
+            float* dst = ring + (size_t)(((uint32_t)r + (uint32_t)hlo) & rmask) * n;
+            for (uint32_t y = tid; y < n; y += bs) dst[y] = 0.0f;
+        }
+        if (zeroed && na >= nb) {
+            __syncthreads();
+        }
+        // C, D: kick the entering rows, at most T at a time through the
+        // staging rows
+        for (int a = na; a < nb; a += (int)T) {
+            const int c = min((int)T, nb - a);
+            const uint32_t tot4 = (uint32_t)c * n4;
+            const gfloat4* src4 = (const gfloat4*)(in_mesh + (size_t)a * n);
+            vec4f* st4 = reinterpret_cast<vec4f*>(stage);
+            for (uint32_t base = 0; base < tot4; base += bs * STRIP_LOAD_BATCH) {
+                vec4f v[STRIP_LOAD_BATCH];
+#pragma unroll
+                for (int b = 0; b < STRIP_LOAD_BATCH; b++) {
+                    const uint32_t i = base + tid + (uint32_t)b * bs;
+                    v[b] = src4[i < tot4 ? i : tot4 - 1u];
+                }
+#pragma unroll
+                for (int b = 0; b < STRIP_LOAD_BATCH; b++) {
+                    const uint32_t i = base + tid + (uint32_t)b * bs;
+                    if (i < tot4) {
+                        vec4f w = v[b];
+                        if (renorm) {
+                            w.x = w.x * norm;
+                            w.y = w.y * norm;
+                            w.z = w.z * norm;
+                            w.w = w.w * norm;
+                        }
+                        st4[i] = w;
+                    }
+                }
+            }
+            __syncthreads();
+            strip_kick<IT>(stage, ring, n, a, c, k0, (uint32_t)hlo, rmask, tapw,
+                           tapr, wake);
+            __syncthreads();
+        }
+        // E: drift along x from the ring into the staging rows
+#pragma unroll
+        for (int c = 0; c < MAXC; c++) {
+            const uint32_t y = tid + (uint32_t)c * bs;
+            if (c >= ncols || y >= n) {
+                continue;
+            }
+            const float* w = col_dw[c];
+            if (dcons & (1u << c)) {
+                // the taps of output row x0 + o are rows r0 + o .. r0 + o + IT - 1
+                const int r0 = x0 + (int)col_d0[c] - half;
+                if (r0 >= wlo && r0 + rows_out + IT - 2 < whi) {
+                    float win[STRIP_MAX_T + IT - 1];
+#pragma unroll
+                    for (int t = 0; t < (int)STRIP_MAX_T + IT - 1; t++) {
+                        if (t < rows_out + IT - 1) {
+                            const uint32_t slot =
+                                ((uint32_t)(r0 + t) + (uint32_t)hlo) & rmask;
+                            win[t] = ring[(size_t)slot * n + y];
+                        }
+                    }
+#pragma unroll
+                    for (int o = 0; o < (int)STRIP_MAX_T; o++) {
+                        if (o < rows_out) {
+                            float v = 0.0f;
+#pragma unroll
+                            for (int j = 0; j < IT; j++) {
+                                v = v + win[o + j] * w[j];
+                            }
+                            stage[(size_t)o * n + y] = v;
+                        }
+                    }
+                    continue;
+                }
+            }
+            // taps not consecutive, or beyond the window (an error)
+            uint32_t idx[IT];
+            {
+                float wdummy[IT];
+                kick_taps<IT>(drift_off[y], n, idx, wdummy);
+            }
+            for (int o = 0; o < rows_out; o++) {
+                const int x = x0 + o;
+                float v = 0.0f;
+#pragma unroll
+                for (int j = 0; j < IT; j++) {
+                    int xs = x + (int)idx[j] - half;
+                    if (xs < wlo || xs >= whi) {
+                        if (xs < 0 || xs >= in) {
+                            continue;  // outside the mesh: skipped
+                        }
+                        atomicOr(d.err, 1u);
+                        xs = xs < wlo ? wlo : whi - 1;
+                    }
+                    const uint32_t slot = ((uint32_t)xs + (uint32_t)hlo) & rmask;
+                    v = v + ring[(size_t)slot * n + y] * w[j];
+                }
+                stage[(size_t)o * n + y] = v;
+            }
+        }
+        __syncthreads();
+        // F: Fokker-Planck along y, the output rows to global memory
+#pragma unroll
+        for (int c = 0; c < MAXC; c++) {
+            const uint32_t y = tid + (uint32_t)c * bs;
+            if (c >= ncols || y >= n) {
+                continue;
+            }
+            const bool fc = (fcons >> c) & 1u;
+            const uint32_t f0 = col_f0[c];
+            uint32_t fi[IPFP > 0 ? IPFP : 1];
+            if (IPFP > 0 && !fc) {
+#pragma unroll
+                for (int kk = 0; kk < IPFP; kk++) {
+                    fi[kk] = fp_i[(size_t)kk * n + y];
+                }
+            }
+            for (int o = 0; o < rows_out; o++) {
+                const float* src = stage + (size_t)o * n;
+                float v;
+                if constexpr (IPFP == 0) {
+                    v = src[y];
+                } else {
+                    v = 0.0f;
+                    if (fc) {
+#pragma unroll
+                        for (int kk = 0; kk < IPFP; kk++) {
+                            v = v + src[f0 + (uint32_t)kk] * col_fw[c][kk];
+                        }
+                    } else {
+#pragma unroll
+                        for (int kk = 0; kk < IPFP; kk++) {
+                            v = v + src[fi[kk]] * col_fw[c][kk];
+                        }
+                    }
+                }
+                out[(size_t)(x0 + o) * n + y] = v;
+            }
+        }
+        __syncthreads();  // the staging rows are reloaded next
+    }
+
+    // ---- H: strip 0 closes the step -----------------------------------------
+    if (strip == 0) {
+        step_close(d, parity, cnt, sh.red);
+    }
+    // ---- G: x projection of the strip's rows --------------------------------
+    // The rows were stored by this workgroup before the barrier above.  In
+    // chunks of C columns, every thread loads them (coalesced) into the
+    // ring and staging rows, free by now, as the products v * w; then one
+    // lane per row adds its chunk, continuing the reference's
+    // std::inner_product order.  Rows in LDS are C + 4 words apart, so the
+    // lanes' reads of one column fall in different banks.
+    {
+        float* pbuf = ring;  // the staging rows follow the ring
+        const uint32_t room = (d.ringw + T) * n;
+        uint32_t C = 4;
+        while (C * 2 <= n && (uint32_t)srows * (C * 2 + 4) <= room) {
+            C *= 2;
+        }
+        const uint32_t pst = C + 4;
+        const uint32_t c4 = C >> 2;
+        const uint32_t lc4 = (uint32_t)__builtin_ctz(c4);
+        const uint32_t tot = (uint32_t)srows << lc4;
+        float acc = 0.0f;
+        for (uint32_t c0 = 0; c0 < n; c0 += C) {
+            const uint32_t cw4 = min(C, n - c0) >> 2;
+            for (uint32_t i = tid; i < tot; i += bs) {
+                const uint32_t r = i >> lc4, q = i & (c4 - 1u);
+                if (q < cw4) {
+                    const vec4f v = *(const gfloat4*)(out + (size_t)(s0 + (int)r) * n +
+                                                      c0 + 4u * q);
+                    const vec4f w =
+                        *reinterpret_cast<const vec4f*>(wsl + c0 + 4u * q);
+                    vec4f p;
+                    p.x = v.x * w.x;
+                    p.y = v.y * w.y;
+                    p.z = v.z * w.z;
+                    p.w = v.w * w.w;
+                    *reinterpret_cast<vec4f*>(pbuf + r * pst + 4u * q) = p;
+                }
+            }
+            __syncthreads();
+            if ((int)tid < srows) {
+                const vec4f* p4 = reinterpret_cast<const vec4f*>(pbuf + tid * pst);
+#pragma unroll 8
+                for (uint32_t q = 0; q < cw4; q++) {
+                    const vec4f p = p4[q];
+                    acc = acc + p.x;
+                    acc = acc + p.y;
+                    acc = acc + p.z;
+                    acc = acc + p.w;
+                }
+            }
+            __syncthreads();
+        }
+        if ((int)tid < srows) {
+            const int x = s0 + (int)tid;
+            proj_out[x] = acc;
+            if (d.prof_hist && d.counter && cnt < d.hist_cap) {
+                prof_hist[(size_t)cnt * n + x] = acc;
+            }
+        }
+    }
+}
+
+// One workgroup per strip of a batch: blockmap[nmap] holds (instance << 16
+// | strip) (STEP_NO_TILE: padding); nmap and the grid are multiples of 8,
+// cut into 8 contiguous runs, one per XCD.
+template <int IT, int IPFP, int MAXC>
+__global__ void __launch_bounds__(STEP_MAX_THREADS)
+strip_kernel(const InstDesc* __restrict__ descs,
+             const uint32_t* __restrict__ blockmap, uint32_t nmap,
+             uint32_t parity)
+{
+    extern __shared__ __align__(16) float lds[];
+    __shared__ StepShared sh;
+    const uint32_t logical = (blockIdx.x & 7u) * (nmap >> 3) + (blockIdx.x >> 3);
+    const uint32_t m = blockmap[logical];
+    if (m == STEP_NO_TILE) {
+        return;
+    }
+    strip_body<IT, IPFP, MAXC>(descs[m >> 16], m & 0xFFFFu, parity, lds, sh);
+}
+
+}  // namespace vfps_impl

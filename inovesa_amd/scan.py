@@ -153,7 +153,7 @@ class Runner:
                 for r in self.tuning:
                     if r["chosen"] and r["n"] == s.n:
                         self.tiles[key] = (r["T"], r["threads"],
-                                           r["persistent"])
+                                           r["persistent"], r["strip"])
             dt = time.perf_counter() - t0
             self.gpu_seconds += dt
             self.tune_seconds += dt
@@ -271,7 +271,7 @@ def tune_tiles(setup, count, gap=0.04, damping=2500.0):
     for r in b.autotune(steps=8):
         if r["chosen"]:
             tiles[(r["n"], batch_class(count))] = (r["T"], r["threads"],
-                                                   r["persistent"])
+                                                   r["persistent"], r["strip"])
     return tiles, time.perf_counter() - t0
 
 

@@ -803,11 +803,12 @@ class Batch:
 
     def config(self, i):
         """Instance i's configuration: fused, wake method, tile rows,
-        threads, LDS buffer rows, LDS bytes, persistent launch."""
-        out = np.zeros(7, np.uint32)
-        call("vfps_batch_config", self.h, i, _p(out), 7)
+        threads, LDS buffer rows, LDS bytes, persistent launch, tiles per
+        strip (0 outside the strip form)."""
+        out = np.zeros(8, np.uint32)
+        call("vfps_batch_config", self.h, i, _p(out), 8)
         return dict(zip(("fused", "wake_method", "tile_rows", "threads",
-                         "stride", "lds_bytes", "persistent"),
+                         "stride", "lds_bytes", "persistent", "strip"),
                         (int(v) for v in out)))
 
     def launches(self):
@@ -825,15 +826,24 @@ class Batch:
         LDS class as parallel branches (False, the default)."""
         call("vfps_batch_set_merge", self.h, 1 if merge else 0)
 
-    def retile_size(self, n, tile_rows, threads, persistent=False):
-        """Set the tiling of every instance of mesh size n."""
-        call("vfps_batch_retile_size", self.h, n, tile_rows, threads,
-             1 if persistent else 0)
+    def retile_size(self, n, tile_rows, threads, persistent=False, strip=0):
+        """Set the tiling of every instance of mesh size n (strip > 0: the
+        strip form, strip tiles per workgroup)."""
+        if strip:
+            call("vfps_batch_retile_size_strip", self.h, n, tile_rows,
+                 threads, strip)
+        else:
+            call("vfps_batch_retile_size", self.h, n, tile_rows, threads,
+                 1 if persistent else 0)
 
-    def retile(self, i, tile_rows, threads, persistent=False):
+    def retile(self, i, tile_rows, threads, persistent=False, strip=0):
         """Set instance i's tiling by hand (results do not depend on it)."""
-        call("vfps_batch_retile", self.h, i, tile_rows, threads,
-             1 if persistent else 0)
+        if strip:
+            call("vfps_batch_retile_strip", self.h, i, tile_rows, threads,
+                 strip)
+        else:
+            call("vfps_batch_retile", self.h, i, tile_rows, threads,
+                 1 if persistent else 0)
 
     def timing(self, nsteps):
         ms = C.c_float()
@@ -841,18 +851,21 @@ class Batch:
         return ms.value
 
     def autotune(self, steps=20):
-        """Pick the tile height, workgroup size and launch form (persistent
-        or one workgroup per tile) per mesh size; returns the timing table
-        (n, T, threads, us/step, chosen, persistent, retimed: the three
-        fastest of a size are timed again over 4 x steps)."""
-        cap = 512
-        table = np.zeros((cap, 7), np.float32)
+        """Pick the tile height, workgroup size and launch form (one
+        workgroup per tile, persistent, or strips of tiles) per mesh size;
+        returns the timing table (n, T, threads, us/step, chosen,
+        persistent, retimed: the three fastest of a size are timed again
+        over 4 x steps; strip: tiles per strip, 0 outside the strip
+        form)."""
+        cap = 1024
+        table = np.zeros((cap, 8), np.float32)
         rows = C.c_uint32()
         call("vfps_batch_autotune", self.h, steps, _p(table), cap,
              C.byref(rows))
         return [dict(n=int(r[0]), T=int(r[1]), threads=int(r[2]),
                      us_per_step=float(r[3]), chosen=bool(r[4]),
-                     persistent=bool(r[5]), retimed=bool(r[6]))
+                     persistent=bool(r[5]), retimed=bool(r[6]),
+                     strip=int(r[7]))
                 for r in table[:min(rows.value, cap)]]
 
     def __del__(self):

@@ -129,6 +129,97 @@ def test_production_batch_equals_single_steps(V):
         assert np.array_equal(b.history(k)[0], s.history()[0]), k
 
 
+@pytest.mark.parametrize("tile,strip,threads", [(4, 8, 512), (2, 8, 256),
+                                                (8, 8, 512), (4, 4, 128),
+                                                (2, 32, 256)])
+def test_strip_form_equals_single_steps(V, tile, strip, threads):
+    """The strip form (a workgroup advances `strip` consecutive tiles of one
+    instance, kicking every row once into a ring of kicked rows, and sums
+    the strip's x projection one row per lane) gives every instance's
+    result bit for bit: mesh, profile, wake offsets, CSR history.  196 rows
+    end in a partial strip and a partial tile."""
+    inst = _instances(V, (128, 196, 384, 512), 2)
+    b = _batch(V, inst)
+    for k in range(len(inst)):
+        b.retile(k, tile, threads, strip=strip)
+        cfg = b.config(k)
+        assert (cfg["tile_rows"], cfg["threads"], cfg["strip"]) == (
+            tile, threads, strip)
+    steps = 70                       # one graph replay and 6 single steps
+    b.record(steps + 8)
+    b.run(steps)
+    for k, i in enumerate(inst):
+        s = _single(V, i)
+        s.record(steps + 8)
+        s.run(steps)
+        for x, y in zip(b.mesh(k), s.mesh()):
+            assert np.array_equal(x, y), (k, i[0]["n"])
+        cb, pb = b.history(k)
+        cs, ps = s.history()
+        assert np.array_equal(cb, cs), k
+
+
+def test_strip_form_beside_rotation_tiles(V):
+    """Kick-drift instances in strips and rotation instances in tiles in
+    one batch step (two launches): each equals its run alone."""
+    inst = _instances(V, (128, 256), 2, scheme="mixed")
+    b = _batch(V, inst)
+    for k, i in enumerate(inst):
+        if i[0]["scheme"] == 0:
+            b.retile(k, 4, 256, strip=8)
+    with pytest.raises(V.VfpsInvalid):
+        b.retile([k for k, i in enumerate(inst) if i[0]["scheme"] == 1][0],
+                 4, 256, strip=8)
+    steps = 66
+    b.record(steps + 4)
+    b.run(steps)
+    for k, i in enumerate(inst):
+        s = _single(V, i)
+        s.record(steps + 4)
+        s.run(steps)
+        for x, y in zip(b.mesh(k), s.mesh()):
+            assert np.array_equal(x, y), k
+        assert np.array_equal(b.history(k)[0], s.history()[0]), k
+
+
+def test_strip_form_rejects_and_keeps_tiling(V):
+    inst = _instances(V, (128,), 2)
+    b = _batch(V, inst)
+    before = b.config(0)
+    for T, K in ((16, 2), (4, 32)):   # tiles above 8 rows, strips above 64
+        with pytest.raises(V.VfpsInvalid):
+            b.retile(0, T, 256, strip=K)
+        assert b.config(0) == before
+    b.retile_size(128, 4, 256, strip=8)
+    assert all(b.config(k)["strip"] == 8 for k in range(len(inst)))
+    b.retile_size(128, 2, 256)
+    assert all(b.config(k)["strip"] == 0 for k in range(len(inst)))
+    b.run(3)
+
+
+def test_production_strip_form_equals_single_steps(V):
+    """Two instances of the 1024^2 production workload (five-point FP
+    stencil, second-order drift) in strips of 8 tiles of 4 rows."""
+    from inovesa_amd.workloads import impedances, initial_mesh, production_config
+    inst = []
+    for k, (f, g) in enumerate([(0.8, 0.024), (1.5, 0.056)]):
+        desc, d, rw = production_config(gap=g, current_factor=f)
+        zw, zc = impedances(d, resistive_wall=rw)
+        inst.append((desc, zw, zc, initial_mesh(desc["n"], seed=40 + k)))
+    b = _batch(V, inst)
+    b.retile_size(1024, 4, 512, strip=8)
+    steps = 66
+    b.record(steps + 4)
+    b.run(steps)
+    for k in range(len(inst)):
+        s = _single(V, inst[k])
+        s.record(steps + 4)
+        s.run(steps)
+        for x, y in zip(b.mesh(k), s.mesh()):
+            assert np.array_equal(x, y), k
+        assert np.array_equal(b.history(k)[0], s.history()[0]), k
+
+
 def test_merged_launch_equals_split(V):
     """One launch for all size classes (every tile with the largest LDS)
     against one per LDS class: the same results bit for bit."""

@@ -143,12 +143,16 @@ def test_renormalize_charge_vs_oracle(V, orc, every):
     assert abs(fill - 1.0) < 1e-3
 
 
-def test_renormalize_batch_equals_single(V):
-    """The batched form renormalises exactly as the single step."""
+@pytest.mark.parametrize("strip", [0, 8])
+def test_renormalize_batch_equals_single(V, strip):
+    """The batched form (tiles, or strips of 8 tiles) renormalises exactly
+    as the single step."""
     desc, d, zw, zc, init = _setup(n=128)
     desc = dict(desc, renormalize=3)
     b = V.Batch([desc, desc], [init, init * np.float32(0.9)], [zw, zw],
                 [zc, zc])
+    if strip:
+        b.retile_size(128, 2, 256, strip=strip)
     b.run(10)
     for k, x0 in enumerate((init, init * np.float32(0.9))):
         s = V.Step(desc, x0, zw, zc)
