@@ -95,8 +95,8 @@ __host__ __device__ inline StripLds strip_lds(uint32_t n, uint32_t T, uint32_t K
 
 // The wake kick and then the RF kick along y (or the RF kick alone) of the
 // rows [a, a + c) staged in LDS, into their ring slots.  The same sums in
-// the same order as kick2_rows / kick_rows; a group of four outputs whose
-// inputs all lie inside the row skips the per-sample checks.
+// the same order as kick2_rows / kick_rows; a wavefront whose groups of
+// four outputs all read inside their rows skips the per-sample checks.
 template <int IT>
 __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
                                            float* __restrict__ ring, uint32_t n,
@@ -129,10 +129,30 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
             const bool inside = lo >= 0 && lo + 2 * IT + 1 < in && clo >= 0 &&
                                 clo + IT + 2 < in;
             float v[2 * IT + 2];
-            if (inside) {
+            if (__all(inside)) {
+                // every group of the wavefront lies inside its row
 #pragma unroll
                 for (int k = 0; k < 2 * IT + 2; k++) {
                     v[k] = s[lo + k];
+                }
+                float cc[IT + 3];
+#pragma unroll
+                for (int k = 0; k < IT + 3; k++) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < IT; j++) {
+                        acc = acc + v[k + j] * w1[j];
+                    }
+                    cc[k] = acc;
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < IT; j++) {
+                        acc = acc + cc[e + j] * w2[j];
+                    }
+                    o[e] = acc;
                 }
             } else {
 #pragma unroll
@@ -140,31 +160,31 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
                     const uint32_t ys = (uint32_t)(lo + k);
                     v[k] = ys < n ? s[ys] : 0.0f;
                 }
-            }
-            float cc[IT + 3];
+                float cc[IT + 3];
 #pragma unroll
-            for (int k = 0; k < IT + 3; k++) {
-                float acc = 0.0f;
+                for (int k = 0; k < IT + 3; k++) {
+                    float acc = 0.0f;
 #pragma unroll
-                for (int j = 0; j < IT; j++) {
-                    acc = acc + v[k + j] * w1[j];
+                    for (int j = 0; j < IT; j++) {
+                        acc = acc + v[k + j] * w1[j];
+                    }
+                    cc[k] = (uint32_t)(clo + k) < n ? acc : 0.0f;
                 }
-                cc[k] = (inside || (uint32_t)(clo + k) < n) ? acc : 0.0f;
-            }
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                float acc = 0.0f;
+                for (int e = 0; e < 4; e++) {
+                    float acc = 0.0f;
 #pragma unroll
-                for (int j = 0; j < IT; j++) {
-                    acc = acc + cc[e + j] * w2[j];
+                    for (int j = 0; j < IT; j++) {
+                        acc = acc + cc[e + j] * w2[j];
+                    }
+                    o[e] = acc;
                 }
-                o[e] = acc;
             }
         } else {
             const int32_t lo = y0 + base2;
             const bool inside = lo >= 0 && lo + IT + 2 < in;
             float v[IT + 3];
-            if (inside) {
+            if (__all(inside)) {
 #pragma unroll
                 for (int k = 0; k < IT + 3; k++) {
                     v[k] = s[lo + k];
@@ -187,7 +207,7 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
             }
         }
         const uint32_t slot = ((uint32_t)r + hlo) & rmask;
-        *reinterpret_cast<vec4f*>(ring + (size_t)slot * n + 4 * g) = o;
+        *reinterpret_cast<vec4f*>(ring + __umul24(slot, n) + 4 * g) = o;
         g += dg;
         row += drow;
         if (g >= ng) {
@@ -474,32 +494,41 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
                 continue;
             }
             const float* w = col_dw[c];
-            if (dcons & (1u << c)) {
-                // the taps of output row x0 + o are rows r0 + o .. r0 + o + IT - 1
-                const int r0 = x0 + (int)col_d0[c] - half;
-                if (r0 >= wlo && r0 + rows_out + IT - 2 < whi) {
-                    float win[STRIP_MAX_T + IT - 1];
+            // the taps of output row x0 + o are rows r0 + o .. r0 + o + IT - 1
+            const int r0 = x0 + (int)col_d0[c] - half;
+            const bool win_ok = ((dcons >> c) & 1u) && r0 >= wlo &&
+                                r0 + rows_out + IT - 2 < whi;
+            auto window = [&]() {
+                float win[STRIP_MAX_T + IT - 1];
 #pragma unroll
-                    for (int t = 0; t < (int)STRIP_MAX_T + IT - 1; t++) {
-                        if (t < rows_out + IT - 1) {
-                            const uint32_t slot =
-                                ((uint32_t)(r0 + t) + (uint32_t)hlo) & rmask;
-                            win[t] = ring[(size_t)slot * n + y];
-                        }
+                for (int t = 0; t < (int)STRIP_MAX_T + IT - 1; t++) {
+                    if (t < rows_out + IT - 1) {
+                        const uint32_t slot =
+                            ((uint32_t)(r0 + t) + (uint32_t)hlo) & rmask;
+                        win[t] = ring[__umul24(slot, n) + y];
                     }
-#pragma unroll
-                    for (int o = 0; o < (int)STRIP_MAX_T; o++) {
-                        if (o < rows_out) {
-                            float v = 0.0f;
-#pragma unroll
-                            for (int j = 0; j < IT; j++) {
-                                v = v + win[o + j] * w[j];
-                            }
-                            stage[(size_t)o * n + y] = v;
-                        }
-                    }
-                    continue;
                 }
+#pragma unroll
+                for (int o = 0; o < (int)STRIP_MAX_T; o++) {
+                    if (o < rows_out) {
+                        float v = 0.0f;
+#pragma unroll
+                        for (int j = 0; j < IT; j++) {
+                            v = v + win[o + j] * w[j];
+                        }
+                        stage[(size_t)o * n + y] = v;
+                    }
+                }
+            };
+            // a wavefront whose columns all read windows takes no per-lane
+            // branch
+            if (__all(win_ok)) {
+                window();
+                continue;
+            }
+            if (win_ok) {
+                window();
+                continue;
             }
             // taps not consecutive, or beyond the window (an error)
             uint32_t idx[IT];
@@ -542,6 +571,24 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
                 for (int kk = 0; kk < IPFP; kk++) {
                     fi[kk] = fp_i[(size_t)kk * n + y];
                 }
+            }
+            if (IPFP == 0 || __all(fc)) {
+                // every column of the wavefront has a consecutive stencil
+                for (int o = 0; o < rows_out; o++) {
+                    const float* src = stage + (size_t)o * n;
+                    float v;
+                    if constexpr (IPFP == 0) {
+                        v = src[y];
+                    } else {
+                        v = 0.0f;
+#pragma unroll
+                        for (int kk = 0; kk < IPFP; kk++) {
+                            v = v + src[f0 + (uint32_t)kk] * col_fw[c][kk];
+                        }
+                    }
+                    out[(size_t)(x0 + o) * n + y] = v;
+                }
+                continue;
             }
             for (int o = 0; o < rows_out; o++) {
                 const float* src = stage + (size_t)o * n;
@@ -634,9 +681,11 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
 
 // One workgroup per strip of a batch: blockmap[nmap] holds (instance << 16
 // | strip) (STEP_NO_TILE: padding); nmap and the grid are multiples of 8,
-// cut into 8 contiguous runs, one per XCD.
+// cut into 8 contiguous runs, one per XCD.  Registers are held to what two
+// workgroups of 512 threads per CU leave (128 VGPRs), or, with four
+// columns per thread (workgroups of at most 256 threads), to 256.
 template <int IT, int IPFP, int MAXC>
-__global__ void __launch_bounds__(STEP_MAX_THREADS)
+__global__ void __launch_bounds__(STEP_MAX_THREADS, MAXC >= 4 ? 2 : 4)
 strip_kernel(const InstDesc* __restrict__ descs,
              const uint32_t* __restrict__ blockmap, uint32_t nmap,
              uint32_t parity)
