@@ -683,7 +683,7 @@ int step_retile(vfps_step* s, uint32_t T, uint32_t threads)
         VFPS_CHECK_ARG(T <= STRIP_MAX_T && T * s->strip <= STRIP_MAX_ROWS &&
                            s->desc.n % 4 == 0 &&
                            (strip_ring(T, s->hmax) + T) * s->desc.n >=
-                               8 * T * s->strip,
+                               16 * T * s->strip,
                        "retile: strips of at most 64 rows in tiles of at most "
                        "8 rows");
     }

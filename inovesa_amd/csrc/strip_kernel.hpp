@@ -10,12 +10,13 @@
 //      wavefront sharing each load of the profile (each row's sum keeps the
 //      lane order and the wave_sum_dpp of the tile kernel, so the bits are
 //      the same), then the tiles' CSR shares;
-//   then per sub-tile:
+//   then per sub-tile, two barriers apart:
 //   C, D  the wake and RF kicks of the rows that enter the drift's window,
 //      from the input rows staged in LDS into a ring of kicked rows (a row
-//      is kicked once per strip, not once per tile that reads it);
+//      is kicked once per strip, not once per tile that reads it), beside
+//      the previous sub-tile's F;
 //   E  the drift along x from the ring, a sliding window of consecutive
-//      rows per column;
+//      rows per column, beside the loads of the next entering rows;
 //   F  Fokker-Planck along y and the store of the output rows;
 //   and at the end:
 //   G  the x projection of the strip's rows, one row per lane of one
@@ -49,15 +50,19 @@ __host__ __device__ inline uint32_t strip_ring_rows(uint32_t T, uint32_t halo)
     return r;
 }
 
-// LDS layout of the strip kernel (4-byte words; FP64 arrays first)
+// LDS layout of the strip kernel (4-byte words; FP64 arrays first).  The
+// FP64 kernel windows serve the dot products before the first kick only,
+// so they share their words with the ring and the staging rows.
 struct StripLds {
-    uint32_t gwin, hwin;   // FP64 windows of the wake and CSR kernels
     uint32_t wsum, csum;   // FP64 dot products of the wake rows, CSR rows
     uint32_t pprev;        // previous x projection
     uint32_t wsl;          // Simpson weights
     uint32_t tapw, tapr;   // kick taps of every kicked row of the strip
     uint32_t ring;         // ring of kicked rows, row r at (r + hlo) % rows
-    uint32_t stage;        // [T][n] input rows, then the drift's output
+    uint32_t stage_a;      // [T][n] input rows a kick reads
+    uint32_t stage_b;      // [T][n] the drift's output rows
+    uint32_t gwin, hwin;   // FP64 windows of the wake and CSR kernels
+    uint32_t shared;       // words of ring + staging rows (or the windows)
     uint32_t words;
 };
 
@@ -67,37 +72,41 @@ __host__ __device__ inline StripLds strip_lds(uint32_t n, uint32_t T, uint32_t K
     StripLds L{};
     const uint32_t S = T * K;
     const uint32_t nk = S + halo;  // rows one strip kicks, at most
+    const uint32_t n4 = step_round4(n);
     uint32_t w = 0;
-    L.gwin = w;
-    w += 2 * (nk + n);
-    L.hwin = w;
-    w += 2 * (S + n);
     L.wsum = w;
     w += 2 * nk;
     L.csum = w;
     w += 2 * S;
     w = step_round4(w);
     L.pprev = w;
-    w += step_round4(n);
+    w += n4;
     L.wsl = w;
-    w += step_round4(n);
+    w += n4;
     L.tapw = w;
     w += STEP_TAP_WORDS * nk;
     L.tapr = w;
     w += STEP_TAP_WORDS * nk;
     L.ring = w;
-    w += strip_ring_rows(T, halo) * step_round4(n);
-    L.stage = w;
-    w += T * step_round4(n);
+    L.stage_a = L.ring + strip_ring_rows(T, halo) * n4;
+    L.stage_b = L.stage_a + T * n4;
+    const uint32_t rows_words = L.stage_b + T * n4 - L.ring;
+    L.gwin = w;
+    L.hwin = L.gwin + 2 * (nk + n);
+    const uint32_t win_words = 2 * (nk + n) + 2 * (S + n);
+    L.shared = rows_words > win_words ? rows_words : win_words;
+    w += L.shared;
     L.words = w;
     return L;
 }
 
 // The wake kick and then the RF kick along y (or the RF kick alone) of the
-// rows [a, a + c) staged in LDS, into their ring slots.  The same sums in
-// the same order as kick2_rows / kick_rows; a wavefront whose groups of
-// four outputs all read inside their rows skips the per-sample checks.
-template <int IT>
+// rows [a, a + c) staged in LDS, into their ring slots.  Each thread makes
+// GW consecutive outputs of a row (GW = 4 or 8; n % GW == 0) from the
+// GW + IT - 1 wake-kicked samples they read, each the same sum in the same
+// order as kick2_rows / kick_rows make it; a wavefront whose groups all
+// read inside their rows skips the per-sample checks.
+template <int IT, int GW>
 __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
                                            float* __restrict__ ring, uint32_t n,
                                            int a, int c, int k0, uint32_t hlo,
@@ -106,8 +115,10 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
                                            const float* __restrict__ tapr,
                                            bool wake)
 {
+    constexpr int NC = GW + IT - 1;      // wake-kicked samples a group reads
+    constexpr int NV = GW + 2 * IT - 2;  // input samples behind them
     const uint32_t bs = blockDim.x;
-    const uint32_t ng = n >> 2;
+    const uint32_t ng = n / GW;
     const uint32_t drow = bs / ng, dg = bs % ng;
     uint32_t row = threadIdx.x / ng;
     uint32_t g = threadIdx.x % ng;
@@ -118,26 +129,26 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
         const vec4f w2 = *reinterpret_cast<const vec4f*>(t2);
         const int32_t base2 = __float_as_int(t2[4]);
         const float* s = stage + (size_t)row * n;
-        const int32_t y0 = (int32_t)(4 * g);
-        vec4f o;
+        const int32_t y0 = (int32_t)(GW * g);
+        float o[GW];
         if (wake) {
             const float* t1 = tapw + (uint32_t)(r - k0) * STEP_TAP_WORDS;
             const vec4f w1 = *reinterpret_cast<const vec4f*>(t1);
             const int32_t base1 = __float_as_int(t1[4]);
             const int32_t clo = y0 + base2;
             const int32_t lo = clo + base1;
-            const bool inside = lo >= 0 && lo + 2 * IT + 1 < in && clo >= 0 &&
-                                clo + IT + 2 < in;
-            float v[2 * IT + 2];
+            const bool inside = lo >= 0 && lo + NV - 1 < in && clo >= 0 &&
+                                clo + NC - 1 < in;
+            float v[NV];
+            float cc[NC];
             if (__all(inside)) {
                 // every group of the wavefront lies inside its row
 #pragma unroll
-                for (int k = 0; k < 2 * IT + 2; k++) {
+                for (int k = 0; k < NV; k++) {
                     v[k] = s[lo + k];
                 }
-                float cc[IT + 3];
 #pragma unroll
-                for (int k = 0; k < IT + 3; k++) {
+                for (int k = 0; k < NC; k++) {
                     float acc = 0.0f;
 #pragma unroll
                     for (int j = 0; j < IT; j++) {
@@ -145,24 +156,14 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
                     }
                     cc[k] = acc;
                 }
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    float acc = 0.0f;
-#pragma unroll
-                    for (int j = 0; j < IT; j++) {
-                        acc = acc + cc[e + j] * w2[j];
-                    }
-                    o[e] = acc;
-                }
             } else {
 #pragma unroll
-                for (int k = 0; k < 2 * IT + 2; k++) {
+                for (int k = 0; k < NV; k++) {
                     const uint32_t ys = (uint32_t)(lo + k);
                     v[k] = ys < n ? s[ys] : 0.0f;
                 }
-                float cc[IT + 3];
 #pragma unroll
-                for (int k = 0; k < IT + 3; k++) {
+                for (int k = 0; k < NC; k++) {
                     float acc = 0.0f;
 #pragma unroll
                     for (int j = 0; j < IT; j++) {
@@ -170,34 +171,34 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
                     }
                     cc[k] = (uint32_t)(clo + k) < n ? acc : 0.0f;
                 }
+            }
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    float acc = 0.0f;
+            for (int e = 0; e < GW; e++) {
+                float acc = 0.0f;
 #pragma unroll
-                    for (int j = 0; j < IT; j++) {
-                        acc = acc + cc[e + j] * w2[j];
-                    }
-                    o[e] = acc;
+                for (int j = 0; j < IT; j++) {
+                    acc = acc + cc[e + j] * w2[j];
                 }
+                o[e] = acc;
             }
         } else {
             const int32_t lo = y0 + base2;
-            const bool inside = lo >= 0 && lo + IT + 2 < in;
-            float v[IT + 3];
+            const bool inside = lo >= 0 && lo + NC - 1 < in;
+            float v[NC];
             if (__all(inside)) {
 #pragma unroll
-                for (int k = 0; k < IT + 3; k++) {
+                for (int k = 0; k < NC; k++) {
                     v[k] = s[lo + k];
                 }
             } else {
 #pragma unroll
-                for (int k = 0; k < IT + 3; k++) {
+                for (int k = 0; k < NC; k++) {
                     const uint32_t ys = (uint32_t)(lo + k);
                     v[k] = ys < n ? s[ys] : 0.0f;
                 }
             }
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
+            for (int e = 0; e < GW; e++) {
                 float acc = 0.0f;
 #pragma unroll
                 for (int j = 0; j < IT; j++) {
@@ -207,7 +208,16 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
             }
         }
         const uint32_t slot = ((uint32_t)r + hlo) & rmask;
-        *reinterpret_cast<vec4f*>(ring + __umul24(slot, n) + 4 * g) = o;
+        float* dst = ring + __umul24(slot, n) + GW * g;
+#pragma unroll
+        for (int q = 0; q < GW / 4; q++) {
+            vec4f ov;
+            ov.x = o[4 * q];
+            ov.y = o[4 * q + 1];
+            ov.z = o[4 * q + 2];
+            ov.w = o[4 * q + 3];
+            *reinterpret_cast<vec4f*>(dst + 4 * q) = ov;
+        }
         g += dg;
         row += drow;
         if (g >= ng) {
@@ -248,7 +258,8 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
     float* tapw = lds + L.tapw;
     float* tapr = lds + L.tapr;
     float* ring = lds + L.ring;
-    float* stage = lds + L.stage;
+    float* stage_a = lds + L.stage_a;
+    float* stage_b = lds + L.stage_b;
     const gfloat* in_mesh = (const gfloat*)d.mesh[parity];
     gfloat* out = (gfloat*)d.mesh[parity ^ 1u];
     const gfloat* proj_in = (const gfloat*)d.proj[parity];
@@ -429,64 +440,115 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
     __syncthreads();
 
     // ---- sub-tiles ----------------------------------------------------------
+    // Two barriers per sub-tile: the kick of the rows entering the drift's
+    // window (from stage_a into the ring) runs beside the previous
+    // sub-tile's Fokker-Planck (stage_b to global memory); after a barrier
+    // the drift (ring to stage_b) runs beside the loads of the next
+    // sub-tile's entering rows into stage_a; then a barrier.  A slot the
+    // kick writes held a row below the drift window it follows, so the
+    // ring needs no more rows than one window.
     const uint32_t n4 = n >> 2;
-    for (uint32_t k = 0; k < K; k++) {
-        const int x0 = s0 + (int)(k * T);
-        if (x0 >= in) break;
-        const int rows_out = min((int)T, in - x0);
-        // the drift's window of kicked rows, and the rows entering it
-        const int wlo = x0 - hlo, whi = x0 + rows_out + hhi;
-        const int na = k == 0 ? k0 : min(k1, x0 + hhi);
-        const int nb = min(k1, whi);
-        // rows of the window outside the mesh read as zeros (they enter
-        // the window once: below the mesh in the first sub-tile, above it
-        // as the window passes the last row)
-        const int zlo = k == 0 ? wlo : max(x0 + hhi, in);
-        bool zeroed = false;
-        for (int r = zlo; r < whi; r++) {
+    // rows [a, a + c) of the input (renormalised when due) into dst
+    auto load_rows = [&](int a, int c, float* dst) {
+        const uint32_t tot4 = (uint32_t)c * n4;
+        const gfloat4* src4 = (const gfloat4*)(in_mesh + (size_t)a * n);
+        vec4f* st4 = reinterpret_cast<vec4f*>(dst);
+        for (uint32_t base = 0; base < tot4; base += bs * STRIP_LOAD_BATCH) {
+            vec4f v[STRIP_LOAD_BATCH];
+#pragma unroll
+            for (int b = 0; b < STRIP_LOAD_BATCH; b++) {
+                const uint32_t i = base + tid + (uint32_t)b * bs;
+                v[b] = src4[i < tot4 ? i : tot4 - 1u];
+            }
+#pragma unroll
+            for (int b = 0; b < STRIP_LOAD_BATCH; b++) {
+                const uint32_t i = base + tid + (uint32_t)b * bs;
+                if (i < tot4) {
+                    vec4f w = v[b];
+                    if (renorm) {
+                        w.x = w.x * norm;
+                        w.y = w.y * norm;
+                        w.z = w.z * norm;
+                        w.w = w.w * norm;
+                    }
+                    st4[i] = w;
+                }
+            }
+        }
+    };
+    auto kick_rows = [&](int a, int c) {
+        if ((n & 7u) == 0u) {
+            strip_kick<IT, 8>(stage_a, ring, n, a, c, k0, (uint32_t)hlo, rmask,
+                              tapw, tapr, wake);
+        } else {
+            strip_kick<IT, 4>(stage_a, ring, n, a, c, k0, (uint32_t)hlo, rmask,
+                              tapw, tapr, wake);
+        }
+    };
+    // ring rows of [lo, hi) outside the mesh read as zeros
+    auto zero_rows = [&](int lo, int hi) {
+        for (int r = lo; r < hi; r++) {
             if (r >= 0 && r < in) continue;
-            zeroed = true;
             float* dst = ring + (size_t)(((uint32_t)r + (uint32_t)hlo) & rmask) * n;
             for (uint32_t y = tid; y < n; y += bs) dst[y] = 0.0f;
         }
-        if (zeroed && na >= nb) {
-            __syncthreads();
-        }
-        // C, D: kick the entering rows, at most T at a time through the
-        // staging rows
-        for (int a = na; a < nb; a += (int)T) {
-            const int c = min((int)T, nb - a);
-            const uint32_t tot4 = (uint32_t)c * n4;
-            const gfloat4* src4 = (const gfloat4*)(in_mesh + (size_t)a * n);
-            vec4f* st4 = reinterpret_cast<vec4f*>(stage);
-            for (uint32_t base = 0; base < tot4; base += bs * STRIP_LOAD_BATCH) {
-                vec4f v[STRIP_LOAD_BATCH];
+    };
+    // F: Fokker-Planck along y of sub-tile x0's rows, to global memory
+    auto fp_rows = [&](int x0, int rows_out) {
 #pragma unroll
-                for (int b = 0; b < STRIP_LOAD_BATCH; b++) {
-                    const uint32_t i = base + tid + (uint32_t)b * bs;
-                    v[b] = src4[i < tot4 ? i : tot4 - 1u];
-                }
+        for (int c = 0; c < MAXC; c++) {
+            const uint32_t y = tid + (uint32_t)c * bs;
+            if (c >= ncols || y >= n) {
+                continue;
+            }
+            const bool fc = (fcons >> c) & 1u;
+            const uint32_t f0 = col_f0[c];
+            if (IPFP == 0 || __all(fc)) {
+                // every column of the wavefront has a consecutive stencil
+                for (int o = 0; o < rows_out; o++) {
+                    const float* src = stage_b + (size_t)o * n;
+                    float v;
+                    if constexpr (IPFP == 0) {
+                        v = src[y];
+                    } else {
+                        v = 0.0f;
 #pragma unroll
-                for (int b = 0; b < STRIP_LOAD_BATCH; b++) {
-                    const uint32_t i = base + tid + (uint32_t)b * bs;
-                    if (i < tot4) {
-                        vec4f w = v[b];
-                        if (renorm) {
-                            w.x = w.x * norm;
-                            w.y = w.y * norm;
-                            w.z = w.z * norm;
-                            w.w = w.w * norm;
+                        for (int kk = 0; kk < IPFP; kk++) {
+                            v = v + src[f0 + (uint32_t)kk] * col_fw[c][kk];
                         }
-                        st4[i] = w;
                     }
+                    out[(size_t)(x0 + o) * n + y] = v;
+                }
+                continue;
+            }
+            uint32_t fi[IPFP > 0 ? IPFP : 1];
+            if (IPFP > 0 && !fc) {
+#pragma unroll
+                for (int kk = 0; kk < IPFP; kk++) {
+                    fi[kk] = fp_i[(size_t)kk * n + y];
                 }
             }
-            __syncthreads();
-            strip_kick<IT>(stage, ring, n, a, c, k0, (uint32_t)hlo, rmask, tapw,
-                           tapr, wake);
-            __syncthreads();
+            for (int o = 0; o < rows_out; o++) {
+                const float* src = stage_b + (size_t)o * n;
+                float v = 0.0f;
+                if (fc) {
+#pragma unroll
+                    for (int kk = 0; kk < IPFP; kk++) {
+                        v = v + src[f0 + (uint32_t)kk] * col_fw[c][kk];
+                    }
+                } else {
+#pragma unroll
+                    for (int kk = 0; kk < IPFP; kk++) {
+                        v = v + src[fi[kk]] * col_fw[c][kk];
+                    }
+                }
+                out[(size_t)(x0 + o) * n + y] = v;
+            }
         }
-        // E: drift along x from the ring into the staging rows
+    };
+    // E: drift along x of sub-tile x0's rows from the ring into stage_b
+    auto drift_rows = [&](int x0, int rows_out) {
+        const int wlo = x0 - hlo, whi = x0 + rows_out + hhi;
 #pragma unroll
         for (int c = 0; c < MAXC; c++) {
             const uint32_t y = tid + (uint32_t)c * bs;
@@ -516,7 +578,7 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
                         for (int j = 0; j < IT; j++) {
                             v = v + win[o + j] * w[j];
                         }
-                        stage[(size_t)o * n + y] = v;
+                        stage_b[(size_t)o * n + y] = v;
                     }
                 }
             };
@@ -550,70 +612,67 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
                         xs = xs < wlo ? wlo : whi - 1;
                     }
                     const uint32_t slot = ((uint32_t)xs + (uint32_t)hlo) & rmask;
-                    v = v + ring[(size_t)slot * n + y] * w[j];
+                    v = v + ring[__umul24(slot, n) + y] * w[j];
                 }
-                stage[(size_t)o * n + y] = v;
+                stage_b[(size_t)o * n + y] = v;
+            }
+        }
+    };
+
+    // the first window: its rows outside the mesh, then the rows below
+    // sub-tile 0's own entering rows, at most T at a time
+    const int ro0 = min((int)T, in - s0);
+    const int p1 = min(k1, s0 + hhi);
+    zero_rows(s0 - hlo, s0 + ro0 + hhi);
+    for (int a = k0; a < p1; a += (int)T) {
+        const int c = min((int)T, p1 - a);
+        load_rows(a, c, stage_a);
+        __syncthreads();
+        kick_rows(a, c);
+        __syncthreads();
+    }
+    // sub-tile k kicks rows [ea, eb): for k = 0, [p1, min(k1, s0 + ro0 + hhi))
+    int ea = p1, eb = min(k1, s0 + ro0 + hhi);
+    if (eb > ea) {
+        load_rows(ea, eb - ea, stage_a);
+    }
+    __syncthreads();
+    int prev_x0 = -1, prev_rows = 0;
+    for (uint32_t k = 0; k < K; k++) {
+        const int x0 = s0 + (int)(k * T);
+        if (x0 >= in) break;
+        const int rows_out = min((int)T, in - x0);
+        const int whi = x0 + rows_out + hhi;
+        // C, D beside the previous sub-tile's F
+        if (prev_x0 >= 0) {
+            fp_rows(prev_x0, prev_rows);
+            // rows above the mesh entering this window
+            zero_rows(max(x0 + hhi, in), whi);
+        }
+        if (eb > ea) {
+            kick_rows(ea, eb - ea);
+        }
+        __syncthreads();
+        // E beside the loads of the next sub-tile's entering rows
+        drift_rows(x0, rows_out);
+        ea = eb = 0;
+        const int x0n = x0 + (int)T;
+        if (k + 1 < K && x0n < in) {
+            const int ron = min((int)T, in - x0n);
+            ea = min(k1, x0n + hhi);
+            eb = min(k1, x0n + ron + hhi);
+            if (eb > ea) {
+                load_rows(ea, eb - ea, stage_a);
             }
         }
         __syncthreads();
-        // F: Fokker-Planck along y, the output rows to global memory
-#pragma unroll
-        for (int c = 0; c < MAXC; c++) {
-            const uint32_t y = tid + (uint32_t)c * bs;
-            if (c >= ncols || y >= n) {
-                continue;
-            }
-            const bool fc = (fcons >> c) & 1u;
-            const uint32_t f0 = col_f0[c];
-            uint32_t fi[IPFP > 0 ? IPFP : 1];
-            if (IPFP > 0 && !fc) {
-#pragma unroll
-                for (int kk = 0; kk < IPFP; kk++) {
-                    fi[kk] = fp_i[(size_t)kk * n + y];
-                }
-            }
-            if (IPFP == 0 || __all(fc)) {
-                // every column of the wavefront has a consecutive stencil
-                for (int o = 0; o < rows_out; o++) {
-                    const float* src = stage + (size_t)o * n;
-                    float v;
-                    if constexpr (IPFP == 0) {
-                        v = src[y];
-                    } else {
-                        v = 0.0f;
-#pragma unroll
-                        for (int kk = 0; kk < IPFP; kk++) {
-                            v = v + src[f0 + (uint32_t)kk] * col_fw[c][kk];
-                        }
-                    }
-                    out[(size_t)(x0 + o) * n + y] = v;
-                }
-                continue;
-            }
-            for (int o = 0; o < rows_out; o++) {
-                const float* src = stage + (size_t)o * n;
-                float v;
-                if constexpr (IPFP == 0) {
-                    v = src[y];
-                } else {
-                    v = 0.0f;
-                    if (fc) {
-#pragma unroll
-                        for (int kk = 0; kk < IPFP; kk++) {
-                            v = v + src[f0 + (uint32_t)kk] * col_fw[c][kk];
-                        }
-                    } else {
-#pragma unroll
-                        for (int kk = 0; kk < IPFP; kk++) {
-                            v = v + src[fi[kk]] * col_fw[c][kk];
-                        }
-                    }
-                }
-                out[(size_t)(x0 + o) * n + y] = v;
-            }
-        }
-        __syncthreads();  // the staging rows are reloaded next
+        prev_x0 = x0;
+        prev_rows = rows_out;
     }
+    if (prev_x0 >= 0) {
+        fp_rows(prev_x0, prev_rows);
+    }
+    __syncthreads();  // the projection below reuses the staging rows
 
     // ---- H: strip 0 closes the step -----------------------------------------
     if (strip == 0) {
@@ -621,26 +680,28 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
     }
     // ---- G: x projection of the strip's rows --------------------------------
     // The rows were stored by this workgroup before the barrier above.  In
-    // chunks of C columns, every thread loads them (coalesced) into the
-    // ring and staging rows, free by now, as the products v * w; then one
-    // lane per row adds its chunk, continuing the reference's
-    // std::inner_product order.  Rows in LDS are C + 4 words apart, so the
-    // lanes' reads of one column fall in different banks.
+    // chunks of C columns they are loaded (coalesced) into the ring and
+    // staging rows, free by now, as the products v * w, two chunks in turn:
+    // while one lane per row adds a chunk, continuing the reference's
+    // std::inner_product order, the other wavefronts load the next.  Rows
+    // in LDS are C + 4 words apart, so the lanes' reads of one column fall
+    // in different banks.
     {
-        float* pbuf = ring;  // the staging rows follow the ring
-        const uint32_t room = (d.ringw + T) * n;
+        const uint32_t room = (d.ringw + 2u * T) * n;
         uint32_t C = 4;
-        while (C * 2 <= n && (uint32_t)srows * (C * 2 + 4) <= room) {
+        while (C * 2 <= n && 2u * (uint32_t)srows * (C * 2 + 4) <= room) {
             C *= 2;
         }
         const uint32_t pst = C + 4;
+        float* pb[2] = {ring, ring + (uint32_t)srows * pst};
         const uint32_t c4 = C >> 2;
         const uint32_t lc4 = (uint32_t)__builtin_ctz(c4);
         const uint32_t tot = (uint32_t)srows << lc4;
-        float acc = 0.0f;
-        for (uint32_t c0 = 0; c0 < n; c0 += C) {
+        // the loading threads: all but the summing wavefront, if any
+        const uint32_t lt0 = bs > 64u ? 64u : 0u;
+        auto load_chunk = [&](uint32_t c0, float* buf) {
             const uint32_t cw4 = min(C, n - c0) >> 2;
-            for (uint32_t i = tid; i < tot; i += bs) {
+            for (uint32_t i = tid - lt0; i < tot; i += bs - lt0) {
                 const uint32_t r = i >> lc4, q = i & (c4 - 1u);
                 if (q < cw4) {
                     const vec4f v = *(const gfloat4*)(out + (size_t)(s0 + (int)r) * n +
@@ -652,12 +713,24 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
                     p.y = v.y * w.y;
                     p.z = v.z * w.z;
                     p.w = v.w * w.w;
-                    *reinterpret_cast<vec4f*>(pbuf + r * pst + 4u * q) = p;
+                    *reinterpret_cast<vec4f*>(buf + r * pst + 4u * q) = p;
                 }
             }
-            __syncthreads();
+        };
+        if (tid >= lt0) {
+            load_chunk(0u, pb[0]);
+        }
+        __syncthreads();
+        float acc = 0.0f;
+        uint32_t j = 0;
+        for (uint32_t c0 = 0; c0 < n; c0 += C, j++) {
+            if (c0 + C < n && tid >= lt0) {
+                load_chunk(c0 + C, pb[(j + 1u) & 1u]);
+            }
             if ((int)tid < srows) {
-                const vec4f* p4 = reinterpret_cast<const vec4f*>(pbuf + tid * pst);
+                const uint32_t cw4 = min(C, n - c0) >> 2;
+                const vec4f* p4 =
+                    reinterpret_cast<const vec4f*>(pb[j & 1u] + tid * pst);
 #pragma unroll 8
                 for (uint32_t q = 0; q < cw4; q++) {
                     const vec4f p = p4[q];
