@@ -741,9 +741,13 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
             const vfps_step* m = b->members[i];
             orig.push_back({m->T, m->threads, m->strip, m->persist});
         }
-        // strips exist for the kick-drift maps with the direct wake
+        // strips exist for the kick-drift maps with the direct wake; below
+        // 512 rows a size class has too few strips to fill the GPU, so the
+        // tuner does not spend time on them there
         const bool strips_ok =
-            strip_kernel_for(b->members[cl.second[0]]->variant, 1) != nullptr;
+            n >= 512 &&
+            strip_kernel_for(b->members[cl.second[0]]->variant, 1) != nullptr &&
+            !b->members[cl.second[0]]->desc.fast_projection;
         std::vector<Cand> cands;
         for (uint32_t thr : {128u, 256u, 512u}) {
             if (ceil_div(n, thr) > (uint32_t)STEP_MAX_COLS) continue;
