@@ -26,7 +26,8 @@ Further keys of the line:
                 cubic clamped rotation, Fokker-Planck);
   production    16 meshes of 1024^2 with the large production maps and
                 impedance;
-  scan          the threshold scan with and without the trained network;
+  scan          the threshold scan with and without the trained network,
+                targets on an uneven mix of 128^2-512^2 meshes;
   cpu_baseline  the oracle (the FP32 C restatement of the reference CPU
                 path), compiled on the host with -O3 -march=native, on one
                 host core: 50 warm-up steps, then the median of 5 runs.
@@ -230,15 +231,26 @@ def production_throughput(steps, log, tiles=None, no_graph=False):
 
 
 def scan_ablation(targets, log):
-    """Threshold scans of `targets` (gap, damping) pairs: the plain dense
-    scan and the scan steered by the trained network."""
+    """Threshold scans of `targets` (gap, damping) pairs on the uneven mix
+    of mesh sizes of the batched scan (each target on one of 128^2-512^2;
+    only sizes on which the explicit Fokker-Planck update of its damping
+    time stays bounded): the plain dense scan and the scan steered by the
+    network trained on runs of that mix."""
+    from collections import Counter
     from inovesa_amd.pipeline import compare, scan_targets
     from inovesa_amd.scan import ScanSetup
-    from inovesa_amd.surrogate import Surrogate
-    res = compare(Surrogate.load(), scan_targets(targets), ScanSetup())
+    from inovesa_amd.surrogate import MIX_MODEL, MIX_SIZES, Surrogate
+    setup = ScanSetup(sizes=MIX_SIZES)
+    tg = scan_targets(targets)
+    res = compare(Surrogate.load(MIX_MODEL), tg, setup)
     log(f"scan: dense {res['dense_gpu_s']:.2f} s, surrogate "
         f"{res['surrogate_gpu_s']:.2f} s, ratio {res['gpu_time_ratio']:.3f}")
-    return {k: v for k, v in res.items() if k != "rows"}
+    out = {k: v for k, v in res.items() if k != "rows"}
+    out["workload"] = ("kick-drift maps, PP-CSR, 500 steps/Ts x 30 periods; "
+                       "targets on an uneven mix of mesh sizes")
+    out["targets_per_size"] = {str(k): v for k, v in sorted(
+        Counter(setup.size_of(g, d) for g, d in tg).items())}
+    return out
 
 
 def copy_calibration(mib):

@@ -2,6 +2,8 @@
 surrogate-steered scan with the plain dense scan.
 
     python -m inovesa_amd.pipeline --runs 8192 --focus 0.6 --out DIR
+    python -m inovesa_amd.pipeline --sizes 128,256,384,512 ...   (a mix of
+        mesh sizes: each (gap, damping) target on one of them)
 
 Steps (all on the GPU through the library):
   1. `runs` VFP runs at random (current, gap, damping) points, batched,
@@ -80,13 +82,18 @@ def main(argv=None):
                          "summary")
     ap.add_argument("--model", default=None,
                     help="load a trained model instead of training")
+    ap.add_argument("--sizes", default=None,
+                    help="comma-separated mesh sizes: each (gap, damping) "
+                         "target runs on one of them (default: every run "
+                         "on 128^2)")
     ap.add_argument("--data", default=None,
                     help="train on the runs of a training_data.npz written "
                          "by an earlier call (labels recomputed from their "
                          "fluctuations with this setup's detector)")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
-    setup = ScanSetup()
+    setup = ScanSetup(sizes=tuple(int(v) for v in args.sizes.split(","))
+                      if args.sizes else None)
     log = lambda m: print(m, flush=True)
     summary = dict(setup=setup.__dict__)
     if args.model:

@@ -32,14 +32,14 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import vfps as V
-from .workloads import derive, initial_mesh, scan_instance
+from .workloads import derive, fp_diffusion_number, initial_mesh, scan_instance
 
 C_LIGHT = 2.99792458e8
 
 
 @dataclass
 class ScanSetup:
-    n: int = 128                 # mesh size of every run
+    n: int = 128                 # mesh size of every run (unless sizes)
     steps_per_ts: int = 500      # time steps per synchrotron period
     periods: int = 30            # synchrotron periods per run
     window: int = 5              # periods at the end seen by the detector
@@ -47,10 +47,31 @@ class ScanSetup:
                                  # (None: 1e-3 kick-drift, 5e-3 rotation)
     seed: int = 1                # initial-noise seed (the same for all runs)
     scheme: str = "kick_drift"   # maps of every run (workloads.scan_instance)
+    sizes: tuple = None          # an uneven mix of mesh sizes: each (gap,
+                                 # damping) target runs on one of them
 
     def __post_init__(self):
         if self.tol is None:
             self.tol = 1e-3 if self.scheme == "kick_drift" else 5e-3
+        if self.sizes:
+            self.sizes = tuple(int(v) for v in self.sizes)
+
+    def size_of(self, gap, damping):
+        """Mesh size of the runs of target (gap, damping): n, or one of
+        `sizes` chosen by a hash of the target, so that every scan (and
+        the training data) puts a target on the same mesh.  Only sizes on
+        which the explicit Fokker-Planck update of this damping time stays
+        bounded are drawn: its diffusion number e1/dp^2 (e1 = 2/damping
+        steps) grows with the square of the mesh size, and above 1/2 the
+        update diverges (kept below 0.45)."""
+        if not self.sizes:
+            return self.n
+        ok = [m for m in self.sizes
+              if fp_diffusion_number(m, 2.0 / float(damping)) < 0.45]
+        ok = ok or [min(self.sizes)]
+        key = (int(round(float(gap) * 1e7)) * 2654435761
+               + int(round(float(damping) * 1e3)) * 40503) & 0xFFFFFFFF
+        return ok[(key >> 7) % len(ok)]
 
     @property
     def steps(self):
@@ -58,7 +79,8 @@ class ScanSetup:
 
 
 def scaling_law(gap, setup):
-    """Scaling-law threshold current (A) for a chamber gap (m)."""
+    """Scaling-law threshold current (A) for a chamber gap (m) (the same on
+    every mesh: the ring does not depend on it)."""
     return derive(grid_size=setup.n, steps_per_ts=setup.steps_per_ts,
                   vacuum_gap=gap)["Ith"]
 
@@ -66,10 +88,11 @@ def scaling_law(gap, setup):
 def instance(factor, gap, damping_steps, setup):
     """One run: current factor * scaling-law threshold, parallel-plates CSR
     with the given full gap (m), damping time in time steps, the maps of
-    setup.scheme (desc, derived, init)."""
-    desc, d = scan_instance(setup.n, factor, gap, damping_steps,
+    setup.scheme, on the target's mesh (desc, derived, init)."""
+    n = setup.size_of(gap, damping_steps)
+    desc, d = scan_instance(n, factor, gap, damping_steps,
                             setup.steps_per_ts, setup.scheme)
-    return desc, d, initial_mesh(setup.n, seed=setup.seed)
+    return desc, d, initial_mesh(n, seed=setup.seed)
 
 
 def detect(csr, setup):
@@ -145,15 +168,19 @@ class Runner:
             # results do not depend on it); the tuning steps count as GPU
             # time of the scan
             t0 = time.perf_counter()
-            key = (s.n, batch_class(len(points)))
-            if key in self.tiles:
-                b.retile_size(s.n, *self.tiles[key])
+            counts = {}
+            for desc in descs:
+                counts[desc["n"]] = counts.get(desc["n"], 0) + 1
+            keys = {n: (n, batch_class(c)) for n, c in counts.items()}
+            if all(k in self.tiles for k in keys.values()):
+                for n, k in keys.items():
+                    b.retile_size(n, *self.tiles[k])
             else:
                 self.tuning = b.autotune(steps=8)
                 for r in self.tuning:
-                    if r["chosen"] and r["n"] == s.n:
-                        self.tiles[key] = (r["T"], r["threads"],
-                                           r["persistent"], r["strip"])
+                    if r["chosen"] and r["n"] in keys:
+                        self.tiles[keys[r["n"]]] = (r["T"], r["threads"],
+                                                    r["persistent"], r["strip"])
             dt = time.perf_counter() - t0
             self.gpu_seconds += dt
             self.tune_seconds += dt

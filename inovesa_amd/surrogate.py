@@ -21,9 +21,14 @@ from . import vfps as V
 from .scan import Runner, ScanSetup
 
 DIMS = (3, 256, 256, 256, 65)
-# weights trained by `python -m inovesa_amd.pipeline` (default arguments)
+# weights trained by `python -m inovesa_amd.pipeline` (default arguments:
+# 128^2 runs), and by the same with --sizes 128,256,384,512 (the uneven mix
+# of the batched scan)
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                              "data", "surrogate.npz")
+MIX_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                         "data", "surrogate_mix.npz")
+MIX_SIZES = (128, 256, 384, 512)
 FACTOR_RANGE = (0.3, 3.0)       # current / scaling-law threshold
 FOCUS_RANGE = (0.75, 1.8)       # where the thresholds of the ranges below lie
 GAP_RANGE = (0.016, 0.064)      # full chamber gap (m)

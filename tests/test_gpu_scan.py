@@ -54,3 +54,29 @@ def test_surrogate_outputs():
     assert logit[0] < 0 < logit[1]
     f = model.predict_threshold(0.04, 2000.0)
     assert 0.8 < f < 1.5
+
+
+def test_surrogate_ablation_on_mesh_mix():
+    """The same ablation on the uneven mix of mesh sizes of the batched
+    scan (each target on one of 128^2-512^2 where the explicit
+    Fokker-Planck update of its damping time stays bounded), with the
+    network trained on runs of that mix (inovesa_amd/data/
+    surrogate_mix.npz)."""
+    from collections import Counter
+    from inovesa_amd.pipeline import compare, scan_targets
+    from inovesa_amd.scan import ScanSetup
+    from inovesa_amd.surrogate import MIX_MODEL, MIX_SIZES, Surrogate
+    setup = ScanSetup(sizes=MIX_SIZES)
+    targets = scan_targets(24, seed=29)
+    sizes = Counter(setup.size_of(g, d) for g, d in targets)
+    assert len(sizes) >= 2, sizes
+    res = compare(Surrogate.load(MIX_MODEL), targets, setup)
+    print(f"\n24 targets on {dict(sizes)}: dense {res['dense_runs']} runs "
+          f"{res['dense_gpu_s']:.2f} s, surrogate {res['surrogate_runs']} runs "
+          f"{res['surrogate_gpu_s']:.2f} s (ratio {res['gpu_time_ratio']:.3f}), "
+          f"max threshold difference {100 * res['max_rel_diff']:.2f}%, "
+          f"{res['confirmed']} confirmed")
+    assert res["max_rel_diff"] <= 0.02
+    assert res["surrogate_runs"] <= res["dense_runs"] / 5
+    assert res["confirmed"] >= 22
+    assert res["gpu_time_ratio"] <= 0.25

@@ -97,3 +97,24 @@ def test_detector():
     assert scan.detect(flat, s)[0] is False
     assert scan.detect(burst, s)[0] is True
     assert scan.detect(np.full(100, np.nan), s)[0] is True
+
+
+def test_mesh_mix_assigns_each_target_one_size():
+    """ScanSetup(sizes=...): a target (gap, damping) always runs on the same
+    mesh (dense scan, steered scan and training data agree), and the
+    targets of a scan spread over every size."""
+    from inovesa_amd.pipeline import scan_targets
+    s = scan.ScanSetup(sizes=(128, 256, 384, 512))
+    t = scan_targets(128)
+    sizes = [s.size_of(g, d) for g, d in t]
+    assert sizes == [s.size_of(g, d) for g, d in t]
+    counts = {n: sizes.count(n) for n in (128, 256, 384, 512)}
+    assert counts[128] >= 16 and counts[256] >= 16, counts
+    assert scan.ScanSetup().size_of(0.03, 2000.0) == 128
+    # only meshes on which the explicit FP update of the damping time
+    # stays bounded (diffusion number below 1/2)
+    from inovesa_amd.workloads import fp_diffusion_number
+    for (g, d), n in zip(t, sizes):
+        assert fp_diffusion_number(n, 2.0 / d) < 0.45
+    assert {s.size_of(0.03, 9000.0 + k) for k in range(200)} == {
+        128, 256, 384, 512}
