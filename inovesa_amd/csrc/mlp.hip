@@ -4,7 +4,7 @@
 // power spectrum and the instability verdict of the VFP solver.
 //
 // Dense layers: y = x W^T + b with W [out][in].  The products run on the
-// matrix cores (MFMA 16x16x16, BF16 operands, FP32 accumulation), one
+// matrix cores (MFMA 16x16x32, BF16 operands, FP32 accumulation), one
 // wavefront per 16x16 output tile, with the bias, GELU and its derivative
 // fused into the store.  Master weights, gradients and Adam moments are
 // FP32; each step writes the BF16 copy the products read.
@@ -23,7 +23,8 @@
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bfloat16, round to nearest even (NaN stays NaN)
@@ -73,9 +74,10 @@ struct GemmArgs {
     uint32_t ldc;
 };
 
-// One wavefront computes a 16x16 tile of C with v_mfma_f32_16x16x16_bf16.
-// Operand layout of the instruction: lane l holds A(l%16, 4(l/16)+0..3) and
-// B(4(l/16)+0..3, l%16); it receives C(4(l/16)+r, l%16), r = 0..3.
+// One wavefront computes a 16x16 tile of C with v_mfma_f32_16x16x32_bf16
+// (gfx950: 32 products of K per instruction).  Operand layout of the
+// instruction: lane l holds A(l%16, 8(l/16)+0..7) and B(8(l/16)+0..7,
+// l%16); it receives C(4(l/16)+r, l%16), r = 0..3.
 template <int EPI>
 __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g)
 {
@@ -87,21 +89,23 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g)
         return;
     }
     const uint32_t ti = (wave / tiles_n) * 16u, tj = (wave % tiles_n) * 16u;
-    const uint32_t r16 = lane & 15u, kq = (lane >> 4) * 4u;
+    const uint32_t r16 = lane & 15u, kq = (lane >> 4) * 8u;
     const uint32_t ia = ti + r16;   // A row of this lane
     const uint32_t jb = tj + r16;   // B column of this lane
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (uint32_t k0 = 0; k0 < g.K; k0 += 16u) {
-        bf16x4 a, b;
+    for (uint32_t k0 = 0; k0 < g.K; k0 += 32u) {
+        u16x8 a, b;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < 8; t++) {
             const uint32_t k = k0 + kq + (uint32_t)t;
             const bool ka = ia < g.M && k < g.K;
             const bool kb = jb < g.N && k < g.K;
-            a[t] = ka ? (short)g.A[(int64_t)ia * g.asi + (int64_t)k * g.ask] : (short)0;
-            b[t] = kb ? (short)g.B[(int64_t)k * g.bsk + (int64_t)jb * g.bsj] : (short)0;
+            a[t] = ka ? g.A[(int64_t)ia * g.asi + (int64_t)k * g.ask] : (uint16_t)0;
+            b[t] = kb ? g.B[(int64_t)k * g.bsk + (int64_t)jb * g.bsj] : (uint16_t)0;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+            0, 0, 0);
     }
     const uint32_t j = tj + r16;
 #pragma unroll
