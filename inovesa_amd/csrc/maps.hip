@@ -7,6 +7,7 @@
 // small table built at construction as in the reference.  Every tap is
 // accumulated in the reference's order with separate multiply and add
 // roundings, so outputs equal the reference CPU path bit for bit.
+#include <algorithm>
 #include <cmath>
 #include <random>
 #include <stdexcept>
@@ -144,7 +145,13 @@ static void launch_kick_it(const vfps_map* m, hipStream_t st, const float* in,
         hipLaunchKernelGGL(kick_y_kernel<IT>, dim3(nb * n), dim3(256), 0, st,
                            nb, n, m->offset, in, out);
     } else {
-        const uint32_t rpb = 16;
+        // rows per block: enough blocks to give every CU a few (a 512^2
+        // mesh in blocks of 16 rows was 64 blocks: 12.4 us), at most 16
+        // rows sharing one column's taps
+        const uint64_t cols = ceil_div(n, 256);
+        const uint64_t want = 1024;
+        const uint32_t rpb = (uint32_t)std::min<uint64_t>(
+            16, std::max<uint64_t>(1, cols * n * nb / want));
         hipLaunchKernelGGL(kick_x_kernel<IT>,
                            dim3(ceil_div(n, 256), ceil_div(n, rpb), nb),
                            dim3(256), 0, st, n, rpb, m->offset, in, out);
