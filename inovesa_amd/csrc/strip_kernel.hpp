@@ -36,8 +36,8 @@ namespace vfps_impl {
 constexpr uint32_t STRIP_MAX_ROWS = 64;
 // most rows of one sub-tile
 constexpr uint32_t STRIP_MAX_T = 8;
-// rows of the direct-form dot products one wavefront sums together
-constexpr int STRIP_DOT_ROWS = 8;
+// most rows of the direct-form dot products one wavefront sums together
+constexpr int STRIP_DOT_ROWS = 10;
 // float4 loads of the input rows one thread keeps in flight
 constexpr int STRIP_LOAD_BATCH = 2;
 
@@ -364,18 +364,19 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
     }
 
     // ---- B: wake offsets of the kicked rows, CSR rows of the output rows ----
-    // A job is up to STRIP_DOT_ROWS consecutive rows of one table: lane l
-    // takes x' = l, l + 64, ... and row r accumulates g[r + n - 1 - x'] p[x']
-    // (the tile kernel's window order), then wave_sum_dpp.
+    // A job is up to STRIP_DOT_ROWS consecutive rows of one table, sized so
+    // that the jobs spread over the wavefronts: lane l takes x' = l, l + 64,
+    // ... and row r accumulates g[r + n - 1 - x'] p[x'] (the tile kernel's
+    // window order), then wave_sum_dpp.
     {
-        const int nwj = wake ? (nk + STRIP_DOT_ROWS - 1) / STRIP_DOT_ROWS : 0;
-        const int ncj = csr ? (srows + STRIP_DOT_ROWS - 1) / STRIP_DOT_ROWS : 0;
+        const int tot_rows = (wake ? nk : 0) + (csr ? srows : 0);
+        const int rj = min(STRIP_DOT_ROWS, max(1, (tot_rows + (int)nw - 1) / (int)nw));
+        const int nwj = wake ? (nk + rj - 1) / rj : 0;
+        const int ncj = csr ? (srows + rj - 1) / rj : 0;
         for (int job = (int)wave; job < nwj + ncj; job += (int)nw) {
             const bool isw = job < nwj;
-            const int r_first = isw ? k0 + STRIP_DOT_ROWS * job
-                                    : s0 + STRIP_DOT_ROWS * (job - nwj);
-            const int rows = isw ? min(STRIP_DOT_ROWS, k1 - r_first)
-                                 : min(STRIP_DOT_ROWS, s0 + srows - r_first);
+            const int r_first = isw ? k0 + rj * job : s0 + rj * (job - nwj);
+            const int rows = isw ? min(rj, k1 - r_first) : min(rj, s0 + srows - r_first);
             const double* tbl =
                 isw ? gwin + (r_first - k0 + in - 1) : hwin + (r_first - s0 + in - 1);
             double acc[STRIP_DOT_ROWS];
@@ -383,23 +384,12 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
             for (int i = 0; i < STRIP_DOT_ROWS; i++) {
                 acc[i] = 0.0;
             }
-            if (rows == STRIP_DOT_ROWS) {
-                for (uint32_t x = lane; x < n; x += 64) {
-                    const double p = (double)pprev[x];
-                    const double* t = tbl - x;
+            for (uint32_t x = lane; x < n; x += 64) {
+                const double p = (double)pprev[x];
+                const double* t = tbl - x;
 #pragma unroll
-                    for (int i = 0; i < STRIP_DOT_ROWS; i++) {
-                        acc[i] += t[i] * p;
-                    }
-                }
-            } else {
-                for (uint32_t x = lane; x < n; x += 64) {
-                    const double p = (double)pprev[x];
-                    const double* t = tbl - x;
-#pragma unroll
-                    for (int i = 0; i < STRIP_DOT_ROWS; i++) {
-                        if (i < rows) acc[i] += t[i] * p;
-                    }
+                for (int i = 0; i < STRIP_DOT_ROWS; i++) {
+                    if (i < rows) acc[i] += t[i] * p;
                 }
             }
             double* dst = isw ? wsum + (r_first - k0) : csum + (r_first - s0);
