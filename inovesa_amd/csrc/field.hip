@@ -26,7 +26,7 @@ field_kernel(FieldParams p)
 
 size_t field_lds_bytes(uint32_t M)
 {
-    return sizeof(double2) * (fft_pow2(M) ? M + M / 2 : 2 * M);
+    return sizeof(double2) * (fft_pow2(M) ? fft_slots(M) + M / 2 : 2 * M);
 }
 
 int launch_field(const FieldParams& p, bool wake, bool csr, uint32_t blocks,

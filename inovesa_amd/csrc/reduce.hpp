@@ -83,6 +83,29 @@ __device__ __forceinline__ T block_sum(T v, T* part)
     return r;
 }
 
+// Sum of a double over a workgroup of NT threads (a multiple of 64, <= 16
+// wavefronts); every thread receives it.  wave_sum_dpp within each
+// wavefront, then every thread adds the wavefront sums in wavefront order.
+// Ends with a barrier, so the scratch can be reused right away.
+template <unsigned NT>
+__device__ __forceinline__ double block_sum_dpp(double v, double* part)
+{
+    static_assert(NT % 64 == 0 && NT <= 1024, "whole wavefronts, at most 16");
+    const unsigned lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    constexpr unsigned nwaves = NT / 64;
+    v = wave_sum_dpp(v);
+    if (lane == 0) {
+        part[wave] = v;
+    }
+    __syncthreads();
+    double t = part[0];
+    for (unsigned w = 1; w < nwaves; w++) {
+        t += part[w];
+    }
+    __syncthreads();
+    return t;
+}
+
 // Two sums over the workgroup in one pass; `part` holds two entries per
 // wavefront (<= 8 wavefronts).  The per-wave and cross-wave orders are those
 // of block_sum.  Ends with a barrier.

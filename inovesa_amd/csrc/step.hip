@@ -243,25 +243,53 @@ fields_kernel(const FieldParams* __restrict__ params, uint32_t parity)
     field_body<WAKE, CSR>(p, 0, flds);
 }
 
+// One instance with its parameters by value: the kernel starts from its
+// arguments instead of a load of the parameter block through a pointer.
+template <bool WAKE, bool CSR>
+__global__ void __launch_bounds__(1024)
+field_one_kernel(FieldParams p, uint32_t parity)
+{
+    extern __shared__ double2 flds[];
+    if (parity && p.proj_alt) {
+        p.proj = p.proj_alt;
+    }
+    p.parity = parity;
+    field_body<WAKE, CSR>(p, 0, flds);
+}
+
+template <bool WAKE, bool CSR>
+static const void* fields_fn(bool one)
+{
+    return one ? reinterpret_cast<const void*>(&field_one_kernel<WAKE, CSR>)
+               : reinterpret_cast<const void*>(&fields_kernel<WAKE, CSR>);
+}
+
 int launch_fields(bool wake, bool csr, const FieldParams* params_d,
                   uint32_t ninst, uint32_t Mmax, uint32_t parity,
-                  hipStream_t st)
+                  hipStream_t st, const FieldParams* params_h)
 {
     const size_t lds = field_lds_bytes(Mmax);
+    const bool one = ninst == 1 && params_h;
     const void* fn;
     if (wake && csr) {
-        fn = reinterpret_cast<const void*>(&fields_kernel<true, true>);
+        fn = fields_fn<true, true>(one);
     } else if (wake) {
-        fn = reinterpret_cast<const void*>(&fields_kernel<true, false>);
+        fn = fields_fn<true, false>(one);
     } else if (csr) {
-        fn = reinterpret_cast<const void*>(&fields_kernel<false, true>);
+        fn = fields_fn<false, true>(one);
     } else {
-        fn = reinterpret_cast<const void*>(&fields_kernel<false, false>);
+        fn = fields_fn<false, false>(one);
     }
     if (lds > 65536) {
         VFPS_HIP(hipFuncSetAttribute(fn,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds));
+    }
+    if (one) {
+        FieldParams p = *params_h;
+        void* args[] = {(void*)&p, (void*)&parity};
+        VFPS_HIP(hipLaunchKernel(fn, dim3(1), dim3(FFT_THREADS), args, lds, st));
+        return VFPS_OK;
     }
     void* args[] = {(void*)&params_d, (void*)&parity};
     VFPS_HIP(hipLaunchKernel(fn, dim3(ninst), dim3(FFT_THREADS), args, lds, st));
@@ -340,7 +368,8 @@ static int step_launch_fused(vfps_step* s, uint32_t parity, hipStream_t st)
 {
     if (!s->direct) {
         int rc = launch_fields(s->desc.use_wake != 0, s->desc.use_csr != 0,
-                               s->fparams_d, 1, s->M, parity, st);
+                               s->fparams_d, 1, s->M, parity, st,
+                               &s->fparams_h);
         if (rc != VFPS_OK) return rc;
     }
     return launch_step_kernel(s->variant, s->inst_d, nullptr, s->tiles(),
@@ -372,7 +401,8 @@ static int step_launch_maps(vfps_step* s, uint32_t parity)
         return step_launch_fused(s, parity, st);
     }
     // per-map path, the loop body of src/main.cpp:948-1053
-    int rc = launch_fields(wake, csr, s->fparams_d, 1, s->M, 0, st);
+    int rc = launch_fields(wake, csr, s->fparams_d, 1, s->M, 0, st,
+                           &s->fparams_h);
     if (rc != VFPS_OK) return rc;
     vfps_mesh *t1 = s->t1, *t2 = s->t2, *t3 = s->t3;
     if (wake) {
@@ -1233,6 +1263,7 @@ int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
     s->inst_h.tstamp = mstamp;
     VFPS_HIP(hipMemcpy(reinterpret_cast<char*>(s->fparams_d) + offsetof(FieldParams, tstamp),
                        &fstamp, sizeof(fstamp), hipMemcpyHostToDevice));
+    s->fparams_h.tstamp = fstamp;
     int rc = step_launch_one(s, s->parity);
     s->parity ^= 1u;
     hipError_t e = hipStreamSynchronize(st);
@@ -1245,6 +1276,7 @@ int vfps_step_phases(vfps_step* s, double* ns_out, uint32_t nout)
     s->inst_h.tstamp = nullptr;
     hipMemcpy(reinterpret_cast<char*>(s->fparams_d) + offsetof(FieldParams, tstamp),
               &nul, sizeof(nul), hipMemcpyHostToDevice);
+    s->fparams_h.tstamp = nullptr;
     hipFree(buf);
     if (rc != VFPS_OK) return rc;
     if (e != hipSuccess) return fail(VFPS_ERR_HIP, hipGetErrorString(e));
@@ -1337,7 +1369,7 @@ int vfps_step_timing(vfps_step* s, uint32_t nsteps, float* ms_wake,
             const uint32_t par = s->parity;
             hipEventRecord(e0, st);
             rc = launch_fields(s->desc.use_wake != 0, s->desc.use_csr != 0,
-                               s->fparams_d, 1, s->M, par, st);
+                               s->fparams_d, 1, s->M, par, st, &s->fparams_h);
             hipEventRecord(e1, st);
             if (rc == VFPS_OK) {
                 rc = launch_step_kernel(s->variant, s->inst_d, nullptr,

@@ -107,10 +107,12 @@ void toeplitz_tables(uint32_t n, uint32_t M, const float2* Z,
                      const float2* Zc, const float* renorm, float fdelta,
                      std::vector<double>* g, std::vector<double>* h);
 
-// Field kernel over `ninst` instances (one workgroup each).
+// Field kernel over `ninst` instances (one workgroup each).  params_h:
+// the host copy of the parameters of a single instance, passed by value,
+// or null to read them from params_d.
 int launch_fields(bool wake, bool csr, const FieldParams* params_d,
                   uint32_t ninst, uint32_t Mmax, uint32_t parity,
-                  hipStream_t st);
+                  hipStream_t st, const FieldParams* params_h = nullptr);
 // close the last step of a direct-form run: its CSR power from the tiles'
 // shares (the next step's closing workgroup does this otherwise)
 int launch_step_finalize(const InstDesc* descs_d, uint32_t ninst,
