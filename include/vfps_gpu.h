@@ -492,8 +492,11 @@ VFPS_API int vfps_step_config(vfps_step* s, uint32_t* out, uint32_t nout);
  * of any mesh sizes (each described as for vfps_step_create, with its own
  * initial mesh and impedances) advance together: one step kernel launch per
  * step covers every row tile of every instance (one launch per kernel
- * variant and workgroup size), captured into a hipGraph.  Each instance
- * equals a fused single-mesh step with the direct wake bit for bit. */
+ * variant and workgroup size), captured into a hipGraph.  An instance's
+ * wake_method 1 selects the FFT wake (one field-kernel launch per step
+ * covers every such instance, a workgroup each), anything else the direct
+ * wake.  Each instance equals a fused single-mesh step with the same wake
+ * form bit for bit. */
 VFPS_API int vfps_batch_create(vfps_dev* dev, uint32_t ninst,
                                const vfps_step_desc* descs,
                                const float* const* inits,
@@ -540,8 +543,8 @@ VFPS_API int vfps_batch_launches(vfps_batch* b, uint32_t* out, uint32_t cap,
                                  uint32_t* ngroups);
 /* Set the tiling of one instance by hand: tile rows, workgroup size and
  * launch form (persistent != 0: as many workgroups as stay resident, each
- * walking several tiles).  Results do not depend on it.  On failure the
- * previous tiling stays. */
+ * walking several tiles; direct wake only).  Results do not depend on it.
+ * On failure the previous tiling stays. */
 VFPS_API int vfps_batch_retile(vfps_batch* b, uint32_t inst, uint32_t tile_rows,
                                uint32_t threads, int persistent);
 /* The strip form for one instance (kick-drift maps with the direct wake):

@@ -1,13 +1,16 @@
 // Many independent meshes of uneven sizes advanced together.
 //
-// Every instance is a fused step with the direct wake (its own meshes,
-// maps, impedance kernels, counters and histories).  The batch launches the
-// step kernel once per step for all instances that share a kernel variant
-// and workgroup size: the workgroups of one launch are the row tiles of all
-// its instances (largest instances first), each reading its instance's
-// descriptor, so instances of 128^2 to 1024^2 cells share the GPU in one
-// grid.  Launches of different variants run as parallel branches.  The
-// steps are captured into a hipGraph and replayed.
+// Every instance is a fused step (its own meshes, maps, impedance kernels,
+// counters and histories), with the direct wake or the FFT wake.  The batch
+// launches the step kernel once per step for all instances that share a
+// kernel variant and workgroup size: the workgroups of one launch are the
+// row tiles of all its instances (largest instances first), each reading
+// its instance's descriptor, so instances of 128^2 to 1024^2 cells share
+// the GPU in one grid.  Launches of different variants run as parallel
+// branches.  The FFT-wake instances' field kernels (wake potential and CSR
+// power, one workgroup per instance, each its own transform length) run as
+// one launch before them.  The steps are captured into a hipGraph and
+// replayed.
 #include <algorithm>
 #include <map>
 #include <memory>
@@ -40,6 +43,14 @@ struct Group {
     hipEvent_t done = nullptr;
 };
 
+// The field kernels of the FFT-wake members with one (wake, CSR) pair: a
+// workgroup per member, ahead of the step kernels of the same step.
+struct FieldLaunch {
+    bool wake = false, csr = false;
+    uint32_t ninst = 0, Mmax = 0;
+    FieldParams* params_d = nullptr;
+};
+
 void group_free(Group& g)
 {
     hipFree(g.descs_d);
@@ -55,6 +66,7 @@ struct vfps_batch {
     vfps_dev* dev = nullptr;
     std::vector<vfps_step*> members;
     std::vector<Group> groups;
+    std::vector<FieldLaunch> fields;
     bool merge = false;   // one launch per variant, workgroup size and form
                           // (LDS classes merged: every tile gets the largest
                           // tile's LDS)
@@ -80,6 +92,40 @@ void free_groups(vfps_batch* b)
 {
     for (auto& g : b->groups) group_free(g);
     b->groups.clear();
+    for (auto& f : b->fields) hipFree(f.params_d);
+    b->fields.clear();
+}
+
+// the FFT-wake members' field launches, one per (wake, CSR) pair
+int build_fields(vfps_batch* b)
+{
+    for (int pair = 0; pair < 4; pair++) {
+        const bool wake = (pair & 1) != 0, csr = (pair & 2) != 0;
+        std::vector<FieldParams> ps;
+        uint32_t Mmax = 0;
+        for (const vfps_step* s : b->members) {
+            if (s->direct || (s->desc.use_wake != 0) != wake ||
+                (s->desc.use_csr != 0) != csr) {
+                continue;
+            }
+            ps.push_back(s->fparams_h);
+            Mmax = std::max(Mmax, s->M);
+        }
+        if (ps.empty()) continue;
+        VFPS_CHECK_ARG(field_lds_bytes(Mmax) <= b->dev->lds_max,
+                       "batch: an FFT length exceeds the LDS of one CU");
+        FieldLaunch f;
+        f.wake = wake;
+        f.csr = csr;
+        f.ninst = (uint32_t)ps.size();
+        f.Mmax = Mmax;
+        b->fields.push_back(f);
+        FieldLaunch& fl = b->fields.back();
+        VFPS_HIP(hipMalloc(&fl.params_d, sizeof(FieldParams) * ps.size()));
+        VFPS_HIP(hipMemcpy(fl.params_d, ps.data(), sizeof(FieldParams) * ps.size(),
+                           hipMemcpyHostToDevice));
+    }
+    return VFPS_OK;
 }
 
 bool same_variant(const StepVariant& a, const StepVariant& c)
@@ -116,6 +162,8 @@ int build_group(vfps_batch* b, Group& g)
     }
     VFPS_CHECK_ARG(g.lds <= b->dev->lds_max,
                    "batch: a tile exceeds the LDS of one CU");
+    VFPS_CHECK_ARG(!(g.persist && !g.variant.direct),
+                   "batch: the resident-grid form needs the direct wake");
     g.nblocks = (uint32_t)map.size();
     g.grid = 0;
     VFPS_HIP(hipMalloc(&g.blockmap_d, sizeof(uint32_t) * map.size()));
@@ -186,7 +234,7 @@ int build_groups(vfps_batch* b)
             VFPS_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
         }
     }
-    return VFPS_OK;
+    return build_fields(b);
 }
 
 // one step of every instance: the first group on the device stream, the
@@ -194,6 +242,12 @@ int build_groups(vfps_batch* b)
 int launch_step(vfps_batch* b, uint32_t parity)
 {
     hipStream_t main = b->dev->stream;
+    // the FFT-wake members' wake potentials, CSR powers and step counts
+    for (const auto& f : b->fields) {
+        int rc = launch_fields(f.wake, f.csr, f.params_d, f.ninst, f.Mmax,
+                               parity, main);
+        if (rc != VFPS_OK) return rc;
+    }
     if (b->groups.size() > 1) {
         VFPS_HIP(hipEventRecord(b->fork, main));
     }
@@ -346,7 +400,7 @@ int vfps_batch_create(vfps_dev* dev, uint32_t ninst,
     for (uint32_t i = 0; i < ninst; i++) {
         vfps_step_desc d = descs[i];
         d.fused = 1;
-        d.wake_method = 2;
+        d.wake_method = d.wake_method == 1 ? 1u : 2u;
         vfps_step* s = nullptr;
         int rc = vfps_step_create(dev, &d, inits[i], zwake ? zwake[i] : nullptr,
                                   zcsr ? zcsr[i] : nullptr, &s);
@@ -395,6 +449,8 @@ int vfps_batch_run(vfps_batch* b, uint32_t nsteps)
     }
     if (nsteps > 0) {
         for (auto& g : b->groups) {
+            // (the FFT form's field kernel has recorded each step's power)
+            if (!g.variant.direct) continue;
             rc = launch_step_finalize(g.descs_d, (uint32_t)g.members.size(),
                                       b->parity, st);
             if (rc != VFPS_OK) return rc;
@@ -744,6 +800,8 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
         // strips exist for the kick-drift maps with the direct wake; below
         // 512 rows a size class has too few strips to fill the GPU, so the
         // tuner does not spend time on them there
+        bool direct = true;  // the resident-grid form needs every member
+        for (uint32_t i : cl.second) direct = direct && b->members[i]->direct;
         const bool strips_ok =
             n >= 512 &&
             strip_kernel_for(b->members[cl.second[0]]->variant, 1) != nullptr &&
@@ -756,7 +814,7 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
             std::vector<std::tuple<uint32_t, bool, uint32_t>> forms;
             for (uint32_t T : {1u, 2u, 4u, 8u, 16u}) {
                 forms.emplace_back(T, false, 0u);
-                forms.emplace_back(T, true, 0u);
+                if (direct) forms.emplace_back(T, true, 0u);
             }
             if (strips_ok) {
                 for (uint32_t T : {2u, 4u, 8u}) {
@@ -901,24 +959,33 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    // restore the state
-    hipStreamSynchronize(st);
+    // restore the state (every copy is attempted; the first failure is
+    // reported)
+    hipError_t re = hipStreamSynchronize(st);
+    auto keep = [&re](hipError_t e) {
+        if (re == hipSuccess) re = e;
+    };
     for (size_t i = 0; i < b->members.size(); i++) {
         vfps_step* s = b->members[i];
         const size_t cells = (size_t)s->desc.n * s->desc.n;
         const vfps_mesh* cur = par ? s->t2 : s->t1;
-        hipMemcpy(cur->data, snap[i].mesh, sizeof(float) * cells,
-                  hipMemcpyDeviceToDevice);
-        hipMemcpy(cur->proj, snap[i].proj, sizeof(float) * s->desc.n,
-                  hipMemcpyDeviceToDevice);
+        keep(hipMemcpy(cur->data, snap[i].mesh, sizeof(float) * cells,
+                       hipMemcpyDeviceToDevice));
+        keep(hipMemcpy(cur->proj, snap[i].proj, sizeof(float) * s->desc.n,
+                       hipMemcpyDeviceToDevice));
         const uint32_t cc[2] = {snap[i].counter, snap[i].counter};
-        hipMemcpy(s->counter, cc, sizeof(cc), hipMemcpyHostToDevice);
-        hipMemset(s->csr_valid, 0xFF, sizeof(uint32_t) * 2);
-        hipFree(snap[i].mesh);
-        hipFree(snap[i].proj);
+        keep(hipMemcpy(s->counter, cc, sizeof(cc), hipMemcpyHostToDevice));
+        if (s->csr_valid) {
+            // (the direct form's record of which step the CSR shares are
+            // from; the FFT form has none)
+            keep(hipMemset(s->csr_valid, 0xFF, sizeof(uint32_t) * 2));
+        }
+        keep(hipFree(snap[i].mesh));
+        keep(hipFree(snap[i].proj));
         s->parity = par;
     }
     if (rc != VFPS_OK) return rc;
+    if (re != hipSuccess) return fail(VFPS_ERR_HIP, hipGetErrorString(re));
     if (nrows) *nrows = rows;
     return build_groups(b);
     VFPS_TRY_END

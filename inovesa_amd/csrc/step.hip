@@ -91,8 +91,8 @@ int step_set_lds_limit(const StepVariant& v, size_t bytes)
     for (int form : {STEP_SINGLE, STEP_PER_TILE, STEP_PERSISTENT}) {
         for (uint32_t maxc : {1u, 2u, (uint32_t)STEP_MAX_COLS}) {
             const void* fn = step_kernel_for(v, form, maxc);
-            if (!fn && form != STEP_SINGLE && !v.direct) {
-                continue;  // batch forms exist for the direct wake only
+            if (!fn && form == STEP_PERSISTENT && !v.direct) {
+                continue;  // the resident-grid form is direct-wake only
             }
             if (!fn) {
                 return fail(VFPS_ERR_UNSUPPORTED, "fused step: no kernel for "

@@ -24,14 +24,19 @@ static const void* step_fn_cols(uint32_t maxc)
         &step_kernel<IT, IPFP, DIRECT, SCHEME, CLAMP, FORM, STEP_MAX_COLS>);
 }
 
-// batches run the direct wake only, so the FFT-wake kernels exist in the
-// one-mesh form only
+// the FFT-wake kernels exist in the one-mesh and per-tile forms (a batch's
+// resident-grid form runs the direct wake only)
 template <int IT, int IPFP, bool DIRECT, int SCHEME, bool CLAMP>
 static const void* step_fn(int form, uint32_t maxc)
 {
     if constexpr (!DIRECT) {
-        if (form != STEP_SINGLE) return nullptr;
-        return step_fn_cols<IT, IPFP, DIRECT, SCHEME, CLAMP, STEP_SINGLE>(maxc);
+        if (form == STEP_SINGLE) {
+            return step_fn_cols<IT, IPFP, DIRECT, SCHEME, CLAMP, STEP_SINGLE>(maxc);
+        }
+        if (form == STEP_PER_TILE) {
+            return step_fn_cols<IT, IPFP, DIRECT, SCHEME, CLAMP, STEP_PER_TILE>(maxc);
+        }
+        return nullptr;
     }
     switch (form) {
     case STEP_SINGLE:

@@ -327,3 +327,62 @@ def test_rotation_and_kick_drift_batch_equals_single_steps(V, persistent):
         for x, y in zip(b.mesh(k), s.mesh()):
             assert np.array_equal(x, y), (k, i[0]["scheme"], i[0]["n"])
         assert np.array_equal(b.history(k)[0], s.history()[0]), k
+
+
+def _mixed_wake_batch(V, sizes, per_size, fft_every=2):
+    """Instances of the batched scan, every fft_every-th with the FFT wake,
+    the rest with the direct wake."""
+    inst = _instances(V, sizes, per_size)
+    descs = [dict(i[0], wake_method=1 if k % fft_every == 0 else 2)
+             for k, i in enumerate(inst)]
+    b = V.Batch(descs, [i[3] for i in inst], [i[1] for i in inst],
+                [i[2] for i in inst])
+    return inst, descs, b
+
+
+def _equals_single_runs(V, b, inst, descs, steps, ks):
+    for k in ks:
+        i, d = inst[k], descs[k]
+        s = V.Step(d, i[3], i[1], i[2])
+        s.record(steps + 8, profiles=True)
+        s.run(steps)
+        for x, y in zip(b.mesh(k), s.mesh()):
+            assert np.array_equal(x, y), k
+        cb, pb = b.history(k)
+        cs, ps = s.history()
+        assert len(cb) == steps, k
+        assert np.array_equal(cb, cs) and np.array_equal(pb, ps), k
+
+
+def test_fft_wake_batch_equals_single_steps(V):
+    """FFT-wake instances (one field-kernel launch over all of them, a
+    workgroup each with its own transform length, ahead of their step
+    kernels) beside direct-wake instances in one batch, over 128^2-512^2:
+    each equals its single-mesh run with the same wake form bit for bit,
+    mesh, profile, wake, profile history and CSR power history."""
+    inst, descs, b = _mixed_wake_batch(V, (128, 256, 384, 512), 2)
+    assert {b.config(k)["wake_method"] for k in range(len(inst))} == {1, 2}
+    # FFT and direct instances never share a step-kernel launch
+    assert b.info()["launch_groups"] >= 2
+    steps = 70                       # one graph replay and 6 single steps
+    b.record(steps + 8, profiles=True)
+    b.run(steps)
+    _equals_single_runs(V, b, inst, descs, steps, range(len(inst)))
+
+
+def test_fft_wake_batch_autotune_and_forms(V):
+    """The autotuner times FFT-wake size classes in the per-tile form only
+    (the resident-grid and strip forms are direct-wake forms), a request
+    for them is refused with the tiling kept, and the tuned batch still
+    equals the single runs bit for bit."""
+    inst, descs, b = _mixed_wake_batch(V, (128, 256), 2, fft_every=1)
+    table = b.autotune(steps=4)
+    assert table and not any(r["persistent"] or r["strip"] for r in table)
+    before = b.config(0)
+    with pytest.raises(V.VfpsInvalid):
+        b.retile(0, 2, 256, persistent=True)
+    assert b.config(0) == before
+    steps = 66
+    b.record(steps + 8, profiles=True)
+    b.run(steps)
+    _equals_single_runs(V, b, inst, descs, steps, (0, len(inst) - 1))
