@@ -133,20 +133,22 @@ def form_label(r):
 
 
 def batched_throughput(steps, log, scheme="kick_drift", tiles=None,
-                       no_graph=False):
+                       no_graph=False, wake_method=2):
     """64 independent meshes of uneven sizes (16 each of 128^2, 256^2,
     384^2, 512^2; currents log-spaced over 0.3-3x threshold; gaps 16-64 mm,
     each its own parallel-plates impedance) advanced together after the
     autotuner picked the tiles: whole-batch cell updates per second.
     scheme "rotation": the maps of the 256^2 rotation workload (wake kick,
     cubic clamped rotation, Fokker-Planck); "kick_drift": those of the
-    512^2 workload."""
+    512^2 workload.  wake_method 1: every instance with the FFT wake (one
+    field-kernel launch per step over all of them), 2: the direct wake."""
     import numpy as np
     from inovesa_amd import Batch
     from inovesa_amd.workloads import batched_scan_config, impedances
     cfg = batched_scan_config(scheme=scheme)
     zs = [impedances(d) for _, d, _ in cfg]
-    b = Batch([dict(c[0], no_graph=int(no_graph)) for c in cfg],
+    b = Batch([dict(c[0], no_graph=int(no_graph), wake_method=wake_method)
+               for c in cfg],
               [c[2] for c in cfg], [z[0] for z in zs], [z[1] for z in zs])
     if tiles:
         for size, (T, thr, persistent, strip) in tiles.items():
@@ -168,16 +170,19 @@ def batched_throughput(steps, log, scheme="kick_drift", tiles=None,
     chosen = {r["n"]: (r["T"], r["threads"], form_label(r),
                        round(r["us_per_step"], 2))
               for r in table if r["chosen"]}
-    log(f"batched ({scheme}): {len(cfg)} instances, {cells} cells, {steps} "
-        f"steps in {dt:.2f} s")
+    log(f"batched ({scheme}, wake {wake_method}): {len(cfg)} instances, "
+        f"{cells} cells, {steps} steps in {dt:.2f} s")
     return {"value": cells * steps / dt, "unit": "cell-updates/s",
             "maps": scheme, "instances": len(cfg),
+            "wake": "FFT field kernel" if wake_method == 1
+                    else "direct (FP64 Toeplitz)",
             "cells_per_step": int(cells),
             "steps": steps, "ms_per_step": dt * 1e3 / steps,
             "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
             "traffic_per_step": (load_traffic("batch_ragged_64",
                                               "bytes_per_step")
-                                 if scheme == "kick_drift" else None),
+                                 if scheme == "kick_drift" and wake_method == 2
+                                 else None),
             "launch": launch_label(steps, no_graph),
             "tiles_chosen": {str(k): v for k, v in chosen.items()}}
 
@@ -299,7 +304,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB and exit")
     ap.add_argument("--only", choices=("batched", "batched_rotation",
-                                       "production"),
+                                       "batched_fft_wake", "production"),
                     help="run only this batch line, with the tiling the "
                          "autotuner last chose (for profiler runs), and "
                          "print its JSON object")
@@ -314,11 +319,14 @@ def main():
                                       tiles=PROFILE_TILES["production"],
                                       no_graph=args.no_graph)
         else:
+            fft = args.only == "batched_fft_wake"
             r = batched_throughput(args.batch_steps, log,
                                    scheme="rotation" if args.only ==
                                    "batched_rotation" else "kick_drift",
-                                   tiles=PROFILE_TILES[args.only],
-                                   no_graph=args.no_graph)
+                                   tiles=PROFILE_TILES["batched" if fft
+                                                       else args.only],
+                                   no_graph=args.no_graph,
+                                   wake_method=1 if fft else 2)
         print(json.dumps({args.only: r}), flush=True)
         return
     if args.gpus != 1:
@@ -441,6 +449,12 @@ def main():
         out["batched"] = batched_throughput(args.batch_steps, log)
         out["batched_rotation"] = batched_throughput(args.batch_steps, log,
                                                      scheme="rotation")
+        # the same 64 instances with the FFT wake, on the direct line's
+        # tiling (the batched radix FFT: one field-kernel workgroup per
+        # instance and step)
+        out["batched_fft_wake"] = batched_throughput(
+            args.batch_steps, log, tiles=PROFILE_TILES["batched"],
+            wake_method=1)
         out["production"] = production_throughput(1000, log)
     if not args.no_scan:
         out["scan"] = scan_ablation(args.scan_targets, log)
