@@ -739,7 +739,7 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
         for (uint32_t i : idx) {
             b->members[i]->persist = pers;
             b->members[i]->strip = strip;
-            int r0 = step_retile(b->members[i], T, thr);
+            int r0 = step_retile(b->members[i], T, thr, false);
             if (r0 != VFPS_OK) return r0;
         }
         vfps_batch sub;
@@ -873,11 +873,11 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
             if (best) {
                 m->persist = best->pers;
                 m->strip = best->strip;
-                rc = step_retile(m, best->T, best->thr);
+                rc = step_retile(m, best->T, best->thr, false);
             } else {
                 m->persist = orig[j].pers;
                 m->strip = orig[j].strip;
-                rc = step_retile(m, orig[j].T, orig[j].thr);
+                rc = step_retile(m, orig[j].T, orig[j].thr, false);
             }
             if (rc != VFPS_OK) break;
         }
@@ -934,7 +934,8 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                 b->members[i]->persist = false;
                 b->members[i]->strip = ch.best_tile.strip;
                 if ((rc = step_retile(b->members[i], ch.best_tile.T,
-                                      ch.best_tile.thr)) != VFPS_OK) break;
+                                      ch.best_tile.thr, false)) != VFPS_OK)
+                    break;
             }
             if (rc != VFPS_OK) break;
             if ((rc = time_all(&t_tile)) != VFPS_OK) break;
@@ -944,7 +945,8 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
                     b->members[i]->persist = true;
                     b->members[i]->strip = 0;
                     if ((rc = step_retile(b->members[i], ch.best.T,
-                                          ch.best.thr)) != VFPS_OK) break;
+                                          ch.best.thr, false)) != VFPS_OK)
+                        break;
                 }
             } else if (table) {
                 // the chosen mark moves to the per-tile tiling
@@ -960,7 +962,7 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     // restore the state (every copy is attempted; the first failure is
-    // reported)
+    // reported): also what the retiles above left pending
     hipError_t re = hipStreamSynchronize(st);
     auto keep = [&re](hipError_t e) {
         if (re == hipSuccess) re = e;

@@ -692,7 +692,7 @@ namespace vfps_impl {
 
 // change the tile height and workgroup size of a fused step (the results
 // do not depend on them)
-int step_retile(vfps_step* s, uint32_t T, uint32_t threads)
+int step_retile(vfps_step* s, uint32_t T, uint32_t threads, bool reset)
 {
     VFPS_CHECK_ARG(s && s->fused, "retile: needs a fused step");
     VFPS_CHECK_ARG(threads % 64 == 0 && threads >= 64 &&
@@ -735,7 +735,7 @@ int step_retile(vfps_step* s, uint32_t T, uint32_t threads)
         step_upload_desc(s);
         return rc;
     }
-    if ((rc = step_reset_pending(s)) != VFPS_OK) return rc;
+    if (reset && (rc = step_reset_pending(s)) != VFPS_OK) return rc;
     return step_drop_graph(s);
 }
 
@@ -768,15 +768,14 @@ step_finalize_kernel(const InstDesc* __restrict__ descs, uint32_t parity)
 
 int step_reset_pending(vfps_step* s)
 {
+    // in stream order, after whatever step is still in flight: both
+    // counter entries hold the current count, and no CSR share is pending
     hipStream_t st = s->dev->stream;
-    VFPS_HIP(hipStreamSynchronize(st));
-    uint32_t c = 0;
-    VFPS_HIP(hipMemcpy(&c, s->counter + (s->parity & 1u), sizeof(uint32_t),
-                       hipMemcpyDeviceToHost));
-    const uint32_t cc[2] = {c, c};
-    VFPS_HIP(hipMemcpy(s->counter, cc, sizeof(cc), hipMemcpyHostToDevice));
+    const uint32_t p = s->parity & 1u;
+    VFPS_HIP(hipMemcpyAsync(s->counter + (p ^ 1u), s->counter + p,
+                            sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
     if (s->csr_valid) {
-        VFPS_HIP(hipMemset(s->csr_valid, 0xFF, sizeof(uint32_t) * 2));
+        VFPS_HIP(hipMemsetAsync(s->csr_valid, 0xFF, sizeof(uint32_t) * 2, st));
     }
     return VFPS_OK;
 }

@@ -181,13 +181,15 @@ struct vfps_step {
 };
 
 namespace vfps_impl {
-int step_retile(vfps_step* s, uint32_t T, uint32_t threads);
+// reset: forget pending CSR shares afterwards (step_reset_pending); a
+// caller that restores the whole state itself passes false
+int step_retile(vfps_step* s, uint32_t T, uint32_t threads, bool reset = true);
 // LDS bytes of the step's tile in the form it runs in
 size_t step_lds_of(const vfps_step* s);
 }  // namespace vfps_impl
 
 namespace vfps_impl {
-// forget pending CSR shares and keep the step count at parity 0 (after a
-// retile, a new mesh or a new history)
+// forget pending CSR shares and make both counter entries the current
+// count (after a retile, a new mesh or a new history), in stream order
 int step_reset_pending(vfps_step* s);
 }  // namespace vfps_impl

@@ -126,9 +126,12 @@ def batch_class(count):
 
 class Runner:
     """Runs sets of (factor, gap, damping) points as one batch and keeps
-    the GPU wall time and the number of VFP runs it spent.  The first batch
-    of each mesh size is autotuned (its tuning time counts as GPU time);
-    later batches of that size reuse the tiling."""
+    the wall time it spent and the number of VFP runs.  That time
+    (gpu_seconds) covers a round's whole cost on the device and its host
+    side: building every run's impedance table and the batch (with the
+    direct wake's Toeplitz tables), tuning, and the runs.  The first batch
+    of each mesh size and batch-size class is autotuned; later batches of
+    that class reuse the tiling."""
 
     def __init__(self, setup=None, dev=None, spectra=False, snapshots=5,
                  tune=True, tiles=None):
@@ -138,6 +141,7 @@ class Runner:
         self.snapshots = snapshots
         self.tune = tune
         self.gpu_seconds = 0.0
+        self.build_seconds = 0.0     # part of gpu_seconds building batches
         self.tune_seconds = 0.0      # part of gpu_seconds spent tuning
         self.runs = 0
         self.tuning = None
@@ -149,6 +153,7 @@ class Runner:
 
     def run(self, points):
         s = self.setup
+        t_build = time.perf_counter()
         descs, inits, zws, zcs, ds = [], [], [], [], []
         for f, g, dmp in points:
             desc, d, init = instance(f, g, dmp, s)
@@ -162,6 +167,10 @@ class Runner:
             zcs.append(zc)
             ds.append(d)
         b = V.Batch(descs, inits, zws, zcs, dev=self.dev)
+        b.dev.sync()
+        dt = time.perf_counter() - t_build
+        self.gpu_seconds += dt
+        self.build_seconds += dt
         if self.tune:
             # tile shape per mesh size and batch-size class (a few runs want
             # short tiles for latency, many runs tall ones for throughput;
