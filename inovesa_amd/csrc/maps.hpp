@@ -22,10 +22,15 @@ RotGeom rot_geom(const vfps_map* m);
 // the inner 2x2 taps (src/SM/RotationMap.cpp:159-172).  src(ok, i0, j0)
 // returns the source sample of tap (i0, j0); for taps out of range (ok ==
 // false) it must return in[0], the sample the reference reads there.
-template <int IT, bool CLAMP, typename Src>
+// rows(i0) returns the IT consecutive source rows from i0 on, `stride`
+// samples apart, if they are all at hand, else null: a stencil wholly
+// inside the mesh and those rows reads its taps without per-tap checks,
+// with the same products in the same order.
+template <int IT, bool CLAMP, typename Src, typename Rows>
 __device__ __forceinline__ float rotate_cell_src(uint32_t n, const RotGeom& g,
                                                  uint32_t x0, uint32_t y0,
-                                                 Src src)
+                                                 Src src, Rows rows,
+                                                 uint32_t stride)
 {
     const float at0 = g.qmin + ((float)x0 * g.d0);
     const float at1 = g.pmin + ((float)y0 * g.d1);
@@ -37,7 +42,34 @@ __device__ __forceinline__ float rotate_cell_src(uint32_t n, const RotGeom& g,
     const uint32_t x1 = vf_f2u(xi), y1 = vf_f2u(yi);
     float v = 0.0f;
     float s11 = 0.0f, s12 = 0.0f, s21 = 0.0f, s22 = 0.0f;
-    if (x1 < n && y1 < n) {
+    constexpr uint32_t LO = (uint32_t)((IT - 1) / 2), HI = (uint32_t)(IT / 2);
+    const float* r0 = nullptr;
+    // (written without x1 + HI: a source position in [-2, -1) casts to
+    // 0xFFFFFFFF, as the reference's cast does, and the sum would wrap)
+    if (n > HI && x1 >= LO && x1 < n - HI && y1 >= LO && y1 < n - HI) {
+        r0 = rows(x1 - LO);
+    }
+    if (r0) {
+        float icq[4], icp[4];
+        coefficients<IT>(xf, icq);
+        coefficients<IT>(yf, icp);
+        const float* p = r0 + (y1 - LO);
+#pragma unroll
+        for (int i1 = 0; i1 < IT; i1++) {
+#pragma unroll
+            for (int j1 = 0; j1 < IT; j1++) {
+                const float w = icq[i1] * icp[j1];
+                const float s = p[(size_t)i1 * stride + (uint32_t)j1];
+                v = v + s * w;
+                if constexpr (CLAMP) {
+                    if (i1 == 1 && j1 == 1) s11 = s;
+                    if (i1 == 1 && j1 == 2) s12 = s;
+                    if (i1 == 2 && j1 == 1) s21 = s;
+                    if (i1 == 2 && j1 == 2) s22 = s;
+                }
+            }
+        }
+    } else if (x1 < n && y1 < n) {
         float icq[4], icp[4];
         coefficients<IT>(xf, icq);
         coefficients<IT>(yf, icp);
@@ -89,9 +121,11 @@ __device__ __forceinline__ float rotate_cell(uint32_t n, const RotGeom& g,
                                              const float* __restrict__ in)
 {
     return rotate_cell_src<IT, CLAMP>(
-        n, g, x0, y0, [&](bool ok, uint32_t i0, uint32_t j0) {
+        n, g, x0, y0,
+        [&](bool ok, uint32_t i0, uint32_t j0) {
             return in[ok ? i0 * n + j0 : 0u];
-        });
+        },
+        [&](uint32_t i0) { return in + (size_t)i0 * n; }, n);
 }
 
 // host restatement of the source position of a rotated cell (the same

@@ -899,7 +899,14 @@ __device__ __forceinline__ void step_tile(const InstDesc& d, uint32_t inst,
                         lr = lr < lr0 ? lr0 : lr1 - 1;
                     }
                     return src[(size_t)lr * n + j0];
-                });
+                },
+                [&](uint32_t i0) -> const float* {
+                    // the IT rows from i0 on, if the buffer holds them all
+                    const int lr = (int)i0 - rbase;
+                    return lr >= lr0 && lr + IT <= lr1 ? src + (size_t)lr * n
+                                                       : nullptr;
+                },
+                n);
             oth[(size_t)orow * n + y0] = v;
             y0 += dy;
             orow += drow;
