@@ -51,12 +51,20 @@ kick_y_kernel(uint32_t nb, uint32_t n, const float* __restrict__ off,
     float* dst = out + (size_t)row * n;
     const int32_t half = (int32_t)(n / 2);
     for (uint32_t y = threadIdx.x; y < n; y += blockDim.x) {
+        // every tap's sample in flight at once (an out-of-range tap loads
+        // sample 0 and is not added)
+        uint32_t ys[IT];
+        float s[IT];
+#pragma unroll
+        for (int j = 0; j < IT; j++) {
+            ys[j] = (uint32_t)((int32_t)(y + idx[j]) - half);
+            s[j] = src[ys[j] < n ? ys[j] : 0u];
+        }
         float v = 0.0f;
 #pragma unroll
         for (int j = 0; j < IT; j++) {
-            uint32_t ys = (uint32_t)((int32_t)(y + idx[j]) - half);
-            if (ys < n) {
-                v = v + src[ys] * w[j];
+            if (ys[j] < n) {
+                v = v + s[j] * w[j];
             }
         }
         dst[y] = v;
@@ -85,12 +93,20 @@ kick_x_kernel(uint32_t n, uint32_t rows_per_block,
     const int32_t half = (int32_t)(n / 2);
     const uint32_t x1 = min(x0 + rows_per_block, n);
     for (uint32_t x = x0; x < x1; x++) {
+        // every tap's sample in flight at once (an out-of-range tap loads
+        // row 0 and is not added)
+        uint32_t xs[IT];
+        float s[IT];
+#pragma unroll
+        for (int j = 0; j < IT; j++) {
+            xs[j] = (uint32_t)((int32_t)(x + idx[j]) - half);
+            s[j] = src[(size_t)(xs[j] < n ? xs[j] : 0u) * n + y];
+        }
         float v = 0.0f;
 #pragma unroll
         for (int j = 0; j < IT; j++) {
-            uint32_t xs = (uint32_t)((int32_t)(x + idx[j]) - half);
-            if (xs < n) {
-                v = v + src[(size_t)xs * n + y] * w[j];
+            if (xs[j] < n) {
+                v = v + s[j] * w[j];
             }
         }
         dst[(size_t)x * n + y] = v;

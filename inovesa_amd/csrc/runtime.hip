@@ -95,6 +95,7 @@ __global__ void outer_kernel(uint32_t nb, uint32_t n, const float* g,
 // then one lane per row adds its row in order from LDS.
 constexpr uint32_t PX_ROWS = 16;
 constexpr uint32_t PX_THREADS = 256;
+constexpr int PX_BATCH = 8;   // float4 loads in flight per thread
 
 __global__ void __launch_bounds__(PX_THREADS)
 project_x_kernel(uint32_t rows, uint32_t n, uint32_t rows_per_wg,
@@ -107,19 +108,43 @@ project_x_kernel(uint32_t rows, uint32_t n, uint32_t rows_per_wg,
     const uint32_t stride = n + 4;
     const float* src = data + (size_t)r0 * n;
     if ((n & 3u) == 0u) {
+        // PX_BATCH float4s of the rows (and their weights) in flight per
+        // thread before any is stored: a load that waits for its own data
+        // would wait for every load before it too
         const uint32_t n4 = n >> 2;
         const float4* s4 = reinterpret_cast<const float4*>(src);
         const float4* w4 = reinterpret_cast<const float4*>(ws);
-        for (uint32_t i = threadIdx.x; i < nr * n4; i += blockDim.x) {
-            const uint32_t r = i / n4, g = i - r * n4;
-            const float4 v = s4[i];
-            const float4 w = w4[g];
-            float4 o;
-            o.x = v.x * w.x;
-            o.y = v.y * w.y;
-            o.z = v.z * w.z;
-            o.w = v.w * w.w;
-            *reinterpret_cast<float4*>(px + r * stride + 4 * g) = o;
+        const uint32_t dr = PX_THREADS / n4, dg = PX_THREADS % n4;
+        uint32_t r = threadIdx.x / n4, g = threadIdx.x % n4;
+        while (r < nr) {
+            float4 v[PX_BATCH], w[PX_BATCH];
+            uint32_t rb[PX_BATCH], gb[PX_BATCH];
+#pragma unroll
+            for (int b = 0; b < PX_BATCH; b++) {
+                rb[b] = r;
+                gb[b] = g;
+                // (past the last row: row 0 again, not stored)
+                const uint32_t rs = r < nr ? r : 0u;
+                v[b] = s4[(size_t)rs * n4 + g];
+                w[b] = w4[g];
+                g += dg;
+                r += dr;
+                if (g >= n4) {
+                    g -= n4;
+                    r++;
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < PX_BATCH; b++) {
+                if (rb[b] < nr) {
+                    float4 o;
+                    o.x = v[b].x * w[b].x;
+                    o.y = v[b].y * w[b].y;
+                    o.z = v[b].z * w[b].z;
+                    o.w = v[b].w * w[b].w;
+                    *reinterpret_cast<float4*>(px + rb[b] * stride + 4 * gb[b]) = o;
+                }
+            }
         }
     } else {
         for (uint32_t i = threadIdx.x; i < nr * n; i += blockDim.x) {
