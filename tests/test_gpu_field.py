@@ -275,3 +275,37 @@ def test_nonpower_of_two_wake_and_csr_vs_oracle(V, orc, nmax):
                                    np.float32(d["qscale"]), np.float32(23e9))
     assert rel_linf(spec, rspec) <= 1e-5
     np.testing.assert_allclose(power, rpower, rtol=2e-5)
+
+
+@pytest.mark.parametrize("nmax", [4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048,
+                                  4096, 8192])
+def test_every_fft_length_vs_oracle(V, orc, nmax):
+    """The field kernel's transform at every power-of-two length it takes
+    (complex length M = nmax/2 from 2 to 4096: radix-8 passes with the
+    radix-4 or radix-2 pass each length needs), on a random complex
+    impedance: the wake potential within 1e-5 relative L-inf of the
+    oracle's, which numpy's FFT confirms (test_toeplitz pins the oracle's
+    transform)."""
+    n = min(nmax // 2 if nmax >= 8 else nmax, 512)
+    d = V.derive(grid_size=n)
+    rng = np.random.default_rng(nmax)
+    table = (rng.standard_normal(nmax)
+             + 1j * rng.standard_normal(nmax)).astype(np.complex64)
+    z = V.Impedance(nmax, np.float32(d["fmax"]), table=table)
+    ps = V.PhaseSpace(n, d["qmin"], d["qmax"], d["qscale"], d["pmin"],
+                      d["pmax"], d["pscale"], data=noisy_mesh(n, nmax))
+    f = V.ElectricField(ps, z, [0], 0, 9e6, d["revolutionpart"],
+                        np.float32(1.0))
+    wp = f.wakePotential()
+    prof = ps.getProjection(0)
+    ref, _ = orc.wake_potential(prof, z.data(), [0], 0, np.float32(1.0))
+    assert rel_linf(wp, ref) <= 1e-5
+    # numpy's transform of the reference's recipe: W_k = Z_k F_k below the
+    # Nyquist bin (which stays zero), Im W_0 dropped, unnormalised C2R
+    # times the scaling over nmax
+    F = np.fft.rfft(np.pad(prof[0].astype(np.float64), (0, nmax - n)))
+    W = np.zeros(nmax // 2 + 1, complex)
+    W[:nmax // 2] = table[:nmax // 2].astype(complex) * F[:nmax // 2]
+    W[0] = W[0].real
+    exact = np.fft.irfft(W, nmax)[:n]
+    assert rel_linf(wp[0], exact) <= 1e-5
