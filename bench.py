@@ -111,7 +111,10 @@ def cpu_baseline(desc, zw, zc, init, budget_s=15.0, warmup=50, runs=5):
             "run_seconds": [round(t, 3) for t in times]}
 
 
-def launch_label(steps, no_graph):
+def launch_label(steps, no_graph, batch=False):
+    """How a run of `steps` steps is launched: a single mesh's last graph
+    also closes the run (last CSR power, error flag); a batch launches its
+    close after the graphs."""
     if no_graph:
         return "host launches"
     full, rest = divmod(steps, 64)
@@ -121,7 +124,8 @@ def launch_label(steps, no_graph):
     if rest:
         parts.append(f"one {rest}-step hipGraph")
     return (" + ".join(parts) + " (captured before the timed region), "
-            "then one finalize launch")
+            + ("then one finalize launch" if batch
+               else "the last of them closing the run"))
 
 
 def form_label(r):
@@ -183,7 +187,7 @@ def batched_throughput(steps, log, scheme="kick_drift", tiles=None,
                                               "bytes_per_step")
                                  if scheme == "kick_drift" and wake_method == 2
                                  else None),
-            "launch": launch_label(steps, no_graph),
+            "launch": launch_label(steps, no_graph, batch=True),
             "tiles_chosen": {str(k): v for k, v in chosen.items()}}
 
 
@@ -231,7 +235,7 @@ def production_throughput(steps, log, tiles=None, no_graph=False):
             "steps": steps, "ms_per_step": dt * 1e3 / steps,
             "roofline_frac": 8.0 * cells * steps / dt / (HBM_PEAK_GBS * 1e9),
             "traffic": load_traffic("batch_production_16x1024"),
-            "launch": launch_label(steps, no_graph),
+            "launch": launch_label(steps, no_graph, batch=True),
             "tiles_chosen": chosen[0] if chosen else None}
 
 
@@ -351,8 +355,11 @@ def main():
     st = Step(dict(desc, fast_projection=fast, no_graph=int(args.no_graph)),
               init, zw, zc)
     st.record(args.warmup + args.steps, profiles=True)
-    st.run(args.warmup)
+    # the timed run's graphs are captured first, so that the warm-up steps
+    # run right before the timed ones
     st.prepare(args.steps)
+    st.prepare(args.warmup)
+    st.run(args.warmup)
     st.dev.sync()
     t0 = time.perf_counter()
     st.run(args.steps)

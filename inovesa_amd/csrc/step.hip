@@ -603,10 +603,14 @@ static int step_drop_graph(vfps_step* s)
 constexpr uint32_t STEP_GRAPH = 64;
 
 // the captured run of `steps` steps from parity p (captured on first use)
-static int step_graph(vfps_step* s, uint32_t steps, uint32_t p,
+// (fin, fused steps: the run ends with this graph, which then also closes
+// it: the last step's CSR power and the error flag into its pinned host
+// copy)
+static int step_graph(vfps_step* s, uint32_t steps, uint32_t p, bool fin,
                       hipGraphExec_t* out)
 {
-    const uint32_t key = steps << 1 | (p & 1u);
+    fin = fin && s->fused;
+    const uint32_t key = steps << 2 | (fin ? 2u : 0u) | (p & 1u);
     auto it = s->graphs.find(key);
     if (it != s->graphs.end()) {
         *out = it->second.second;
@@ -619,14 +623,19 @@ static int step_graph(vfps_step* s, uint32_t steps, uint32_t p,
     }
     hipStream_t st = s->dev->stream;
     VFPS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (uint32_t k = 0; k < steps; k++) {
-        int rc = step_launch_one(s, s->fused ? ((p + k) & 1u) : 0u);
-        if (rc != VFPS_OK) {
-            hipGraph_t g = nullptr;
-            hipStreamEndCapture(st, &g);
-            if (g) hipGraphDestroy(g);
-            return rc;
-        }
+    int rc = VFPS_OK;
+    for (uint32_t k = 0; k < steps && rc == VFPS_OK; k++) {
+        rc = step_launch_one(s, s->fused ? ((p + k) & 1u) : 0u);
+    }
+    if (rc == VFPS_OK && fin) {
+        rc = launch_step_finalize(s->inst_d, 1, (p + steps) & 1u, st, s->err_h,
+                                  s->direct);
+    }
+    if (rc != VFPS_OK) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(st, &g);
+        if (g) hipGraphDestroy(g);
+        return rc;
     }
     hipGraph_t def = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -742,11 +751,15 @@ int step_retile(vfps_step* s, uint32_t T, uint32_t threads, bool reset)
 // The CSR power of the last step of a direct-form run (the shares at
 // parity^1), one workgroup per instance; the counter is at parity.
 __global__ void __launch_bounds__(256)
-step_finalize_kernel(const InstDesc* __restrict__ descs, uint32_t parity)
+step_finalize_kernel(const InstDesc* __restrict__ descs, uint32_t parity,
+                     uint32_t* err_host, uint32_t close_csr)
 {
     __shared__ double red[16];
     const InstDesc& d = descs[blockIdx.x];
-    if (!d.use_csr || !d.counter) {
+    if (err_host && threadIdx.x == 0) {
+        err_host[blockIdx.x] = *d.err;
+    }
+    if (!close_csr || !d.use_csr || !d.counter) {
         return;
     }
     const uint32_t cnt = d.counter[parity];
@@ -781,10 +794,11 @@ int step_reset_pending(vfps_step* s)
 }
 
 int launch_step_finalize(const InstDesc* descs_d, uint32_t ninst,
-                         uint32_t parity, hipStream_t st)
+                         uint32_t parity, hipStream_t st, uint32_t* err_host,
+                         bool close_csr)
 {
     hipLaunchKernelGGL(step_finalize_kernel, dim3(ninst), dim3(256), 0, st,
-                       descs_d, parity);
+                       descs_d, parity, err_host, close_csr ? 1u : 0u);
     VFPS_HIP(hipGetLastError());
     return VFPS_OK;
 }
@@ -983,18 +997,21 @@ int vfps_step_run(vfps_step* s, uint32_t nsteps)
     uint32_t left = nsteps;
     int rc = step_flush_desc(s);
     if (rc != VFPS_OK) return rc;
+    bool closed = false;
     if (!s->desc.no_graph) {
         // captured runs: full ones of STEP_GRAPH steps (even, so the parity
-        // stays), then one of the remaining steps
+        // stays), then one of the remaining steps; the last one also closes
+        // the run (fused steps)
         const uint32_t p = s->fused ? (s->parity & 1u) : 0u;
         while (left > 0) {
             const uint32_t k = left >= STEP_GRAPH ? STEP_GRAPH : left;
             hipGraphExec_t g = nullptr;
-            if ((rc = step_graph(s, k, p, &g)) != VFPS_OK) return rc;
+            if ((rc = step_graph(s, k, p, k == left, &g)) != VFPS_OK) return rc;
             VFPS_HIP(hipGraphLaunch(g, st));
             left -= k;
             if (s->fused && (k & 1u)) s->parity ^= 1u;
         }
+        closed = s->fused && nsteps > 0;
     }
     while (left > 0) {
         if ((rc = step_launch_one(s, s->fused ? s->parity : 0)) != VFPS_OK)
@@ -1002,13 +1019,18 @@ int vfps_step_run(vfps_step* s, uint32_t nsteps)
         if (s->fused) s->parity ^= 1u;
         left--;
     }
-    if (s->direct && nsteps > 0) {
-        if ((rc = launch_step_finalize(s->inst_d, 1, s->parity, st)) != VFPS_OK)
+    if (!closed && s->fused && nsteps > 0) {
+        // the same close, launched from here
+        if ((rc = launch_step_finalize(s->inst_d, 1, s->parity, st, s->err_h,
+                                       s->direct)) != VFPS_OK)
             return rc;
+        closed = true;
     }
-    // the error flag comes back in the same stream: one wait for both
-    VFPS_HIP(hipMemcpyAsync(s->err_h, s->err, sizeof(uint32_t),
-                            hipMemcpyDeviceToHost, st));
+    if (!closed) {
+        // the error flag comes back in the same stream: one wait for both
+        VFPS_HIP(hipMemcpyAsync(s->err_h, s->err, sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, st));
+    }
     VFPS_HIP(hipStreamSynchronize(st));
     const uint32_t err = *s->err_h;
     if (err) {
@@ -1026,14 +1048,16 @@ int vfps_step_prepare(vfps_step* s, uint32_t nsteps)
     if (s->desc.no_graph || nsteps == 0) {
         return VFPS_OK;
     }
+    // the graphs vfps_step_run(s, nsteps) replays: full ones that do not end
+    // the run, and the one that ends it
     const uint32_t p = s->fused ? (s->parity & 1u) : 0u;
+    const uint32_t full = nsteps / STEP_GRAPH, rem = nsteps % STEP_GRAPH;
     hipGraphExec_t g = nullptr;
-    if (nsteps >= STEP_GRAPH &&
-        (rc = step_graph(s, STEP_GRAPH, p, &g)) != VFPS_OK) {
+    if ((full > 1 || (full == 1 && rem)) &&
+        (rc = step_graph(s, STEP_GRAPH, p, false, &g)) != VFPS_OK) {
         return rc;
     }
-    if (nsteps % STEP_GRAPH &&
-        (rc = step_graph(s, nsteps % STEP_GRAPH, p, &g)) != VFPS_OK) {
+    if ((rc = step_graph(s, rem ? rem : STEP_GRAPH, p, true, &g)) != VFPS_OK) {
         return rc;
     }
     VFPS_HIP(hipStreamSynchronize(s->dev->stream));

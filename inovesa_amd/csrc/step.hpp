@@ -113,10 +113,13 @@ void toeplitz_tables(uint32_t n, uint32_t M, const float2* Z,
 int launch_fields(bool wake, bool csr, const FieldParams* params_d,
                   uint32_t ninst, uint32_t Mmax, uint32_t parity,
                   hipStream_t st, const FieldParams* params_h = nullptr);
-// close the last step of a direct-form run: its CSR power from the tiles'
-// shares (the next step's closing workgroup does this otherwise)
+// close a run: with close_csr, the last step's CSR power from the tiles'
+// shares (direct form; the next step's closing workgroup does this
+// otherwise); with err_host, every instance's error flag into that pinned
+// host array, so the host reads it after the stream's last wait
 int launch_step_finalize(const InstDesc* descs_d, uint32_t ninst,
-                         uint32_t parity, hipStream_t st);
+                         uint32_t parity, hipStream_t st,
+                         uint32_t* err_host = nullptr, bool close_csr = true);
 
 // exact halo of a kick along x (drift) with the given offsets
 void kick_halo(const std::vector<float>& off, uint32_t n, int it,
