@@ -40,6 +40,8 @@ constexpr uint32_t STRIP_MAX_T = 8;
 constexpr int STRIP_DOT_ROWS = 10;
 // float4 loads of the input rows one thread keeps in flight
 constexpr int STRIP_LOAD_BATCH = 2;
+// float4 loads of stored rows one thread keeps in flight in the projection
+constexpr int STRIP_PROJ_BATCH = 4;
 
 // rows of the ring of kicked rows: a power of two that holds the drift's
 // window of T + halo rows
@@ -689,21 +691,35 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
         const uint32_t tot = (uint32_t)srows << lc4;
         // the loading threads: all but the summing wavefront, if any
         const uint32_t lt0 = bs > 64u ? 64u : 0u;
+        const uint32_t nl = bs - lt0;
+        // a thread's loads of one chunk are issued together; a load with
+        // nothing to fetch reads the chunk's first word of row 0 instead
         auto load_chunk = [&](uint32_t c0, float* buf) {
             const uint32_t cw4 = min(C, n - c0) >> 2;
-            for (uint32_t i = tid - lt0; i < tot; i += bs - lt0) {
-                const uint32_t r = i >> lc4, q = i & (c4 - 1u);
-                if (q < cw4) {
-                    const vec4f v = *(const gfloat4*)(out + (size_t)(s0 + (int)r) * n +
-                                                      c0 + 4u * q);
-                    const vec4f w =
-                        *reinterpret_cast<const vec4f*>(wsl + c0 + 4u * q);
-                    vec4f p;
-                    p.x = v.x * w.x;
-                    p.y = v.y * w.y;
-                    p.z = v.z * w.z;
-                    p.w = v.w * w.w;
-                    *reinterpret_cast<vec4f*>(buf + r * pst + 4u * q) = p;
+            const gfloat* src = out + (size_t)s0 * n + c0;
+            for (uint32_t i0 = tid - lt0; i0 < tot; i0 += STRIP_PROJ_BATCH * nl) {
+                vec4f v[STRIP_PROJ_BATCH];
+#pragma unroll
+                for (int b = 0; b < STRIP_PROJ_BATCH; b++) {
+                    const uint32_t i = i0 + (uint32_t)b * nl;
+                    const uint32_t r = i >> lc4, q = i & (c4 - 1u);
+                    const bool ok = i < tot && q < cw4;
+                    v[b] = *(const gfloat4*)(src + (ok ? (size_t)r * n + 4u * q : 0u));
+                }
+#pragma unroll
+                for (int b = 0; b < STRIP_PROJ_BATCH; b++) {
+                    const uint32_t i = i0 + (uint32_t)b * nl;
+                    const uint32_t r = i >> lc4, q = i & (c4 - 1u);
+                    if (i < tot && q < cw4) {
+                        const vec4f w =
+                            *reinterpret_cast<const vec4f*>(wsl + c0 + 4u * q);
+                        vec4f p;
+                        p.x = v[b].x * w.x;
+                        p.y = v[b].y * w.y;
+                        p.z = v[b].z * w.z;
+                        p.w = v[b].w * w.w;
+                        *reinterpret_cast<vec4f*>(buf + r * pst + 4u * q) = p;
+                    }
                 }
             }
         };
