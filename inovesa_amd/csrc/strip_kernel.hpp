@@ -42,6 +42,12 @@ constexpr int STRIP_DOT_ROWS = 10;
 constexpr int STRIP_LOAD_BATCH = 2;
 // float4 loads of stored rows one thread keeps in flight in the projection
 constexpr int STRIP_PROJ_BATCH = 4;
+// loads per thread of each strip table (profile, weights, kernel windows)
+// issued together before any is stored
+constexpr int STRIP_PRE = 3;
+// float4 loads per thread of the first drift window's rows issued with the
+// tables (when the rows fit the two staging buffers)
+constexpr int STRIP_EARLY4 = 4;
 
 // rows of the ring of kicked rows: a power of two that holds the drift's
 // window of T + halo rows
@@ -281,24 +287,45 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
     const bool csr = d.use_csr != 0;
 
     // ---- A: the strip's tables ---------------------------------------------
+    // The first STRIP_PRE loads per thread of every table, the per-column
+    // tables and the rows the first drift window kicks are all issued before
+    // any is stored, so that they share one memory round trip.
     uint32_t cnt_v = 0;
     if (tid == 0 && d.counter) {
         cnt_v = d.counter[parity];
     }
-    for (uint32_t x = tid; x < n; x += bs) {
-        pprev[x] = proj_in[x];
-        wsl[x] = ws_g[x];
+    const uint32_t n4 = n >> 2;
+    // the first window kicks rows [k0, e0); they go to stage_a and on into
+    // stage_b, which the kernel windows below do not overlap
+    const int ro0 = min((int)T, in - s0);
+    const int e0 = min(k1, s0 + ro0 + hhi);
+    const uint32_t etot4 = (uint32_t)(e0 - k0) * n4;
+    const bool early = e0 - k0 <= 2 * (int)T &&
+                       etot4 <= (uint32_t)STRIP_EARLY4 * bs &&
+                       L.hwin + 2u * (S + n) <= L.stage_a;
+    vec4f er[STRIP_EARLY4];
+    if (early) {
+        const gfloat4* src4 = (const gfloat4*)(in_mesh + (size_t)k0 * n);
+#pragma unroll
+        for (int b = 0; b < STRIP_EARLY4; b++) {
+            const uint32_t i = tid + (uint32_t)b * bs;
+            er[b] = src4[i < etot4 ? i : etot4 - 1u];
+        }
     }
     // the kernel windows the strip's dot products read: row r of the wake
     // needs g[r + n - 1 - x'] (r in [k0, k1)), CSR row r h[r + n - 1 - x']
     // (r in [s0, s0 + srows))
-    if (wake) {
-        const uint32_t gl = (uint32_t)nk + n - 1u;
-        for (uint32_t k = tid; k < gl; k += bs) gwin[k] = wake_g[(uint32_t)k0 + k];
-    }
-    if (csr) {
-        const uint32_t hl = (uint32_t)srows + n - 1u;
-        for (uint32_t k = tid; k < hl; k += bs) hwin[k] = csr_h[(uint32_t)s0 + k];
+    const uint32_t gl = wake ? (uint32_t)nk + n - 1u : 0u;
+    const uint32_t hl = csr ? (uint32_t)srows + n - 1u : 0u;
+    float pv[STRIP_PRE], wv[STRIP_PRE];
+    double gv[STRIP_PRE], hv[STRIP_PRE];
+#pragma unroll
+    for (int b = 0; b < STRIP_PRE; b++) {
+        const uint32_t i = tid + (uint32_t)b * bs;
+        pv[b] = i < n ? proj_in[i] : 0.0f;
+        wv[b] = i < n ? ws_g[i] : 0.0f;
+        gv[b] = i < gl ? wake_g[(uint32_t)k0 + i] : 0.0;
+        hv[b] = i < hl ? csr_h[(uint32_t)s0 + i] : 0.0;
     }
     // per-column tables: the drift taps' first row and weights (the taps
     // are consecutive unless one falls outside the mesh), the
@@ -340,6 +367,30 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
     for (int i = (int)tid; i < nk; i += (int)bs) {
         store_row_taps<IT>(rf_off[k0 + i], n, tapr + (uint32_t)i * STEP_TAP_WORDS);
     }
+#pragma unroll
+    for (int b = 0; b < STRIP_PRE; b++) {
+        const uint32_t i = tid + (uint32_t)b * bs;
+        if (i < n) {
+            pprev[i] = pv[b];
+            wsl[i] = wv[b];
+        }
+        if (i < gl) gwin[i] = gv[b];
+        if (i < hl) hwin[i] = hv[b];
+    }
+    for (uint32_t i = tid + STRIP_PRE * bs; i < n; i += bs) {
+        pprev[i] = proj_in[i];
+        wsl[i] = ws_g[i];
+    }
+    for (uint32_t i = tid + STRIP_PRE * bs; i < gl; i += bs) gwin[i] = wake_g[(uint32_t)k0 + i];
+    for (uint32_t i = tid + STRIP_PRE * bs; i < hl; i += bs) hwin[i] = csr_h[(uint32_t)s0 + i];
+    if (early) {
+        vec4f* st4 = reinterpret_cast<vec4f*>(stage_a);
+#pragma unroll
+        for (int b = 0; b < STRIP_EARLY4; b++) {
+            const uint32_t i = tid + (uint32_t)b * bs;
+            if (i < etot4) st4[i] = er[b];
+        }
+    }
     if (tid == 0) {
         sh.cnt = cnt_v;
     }
@@ -363,6 +414,19 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
         }
         __syncthreads();
         norm = sh.norm;
+        if (early) {
+            // the first window's rows, staged before the factor was known;
+            // each thread scales the words it stored
+            vec4f* st4 = reinterpret_cast<vec4f*>(stage_a);
+            for (uint32_t i = tid; i < etot4; i += bs) {
+                vec4f w = st4[i];
+                w.x = w.x * norm;
+                w.y = w.y * norm;
+                w.z = w.z * norm;
+                w.w = w.w * norm;
+                st4[i] = w;
+            }
+        }
     }
 
     // ---- B: wake offsets of the kicked rows, CSR rows of the output rows ----
@@ -439,7 +503,6 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
     // sub-tile's entering rows into stage_a; then a barrier.  A slot the
     // kick writes held a row below the drift window it follows, so the
     // ring needs no more rows than one window.
-    const uint32_t n4 = n >> 2;
     // rows [a, a + c) of the input (renormalised when due) into dst
     auto load_rows = [&](int a, int c, float* dst) {
         const uint32_t tot4 = (uint32_t)c * n4;
@@ -611,24 +674,31 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
         }
     };
 
-    // the first window: its rows outside the mesh, then the rows below
-    // sub-tile 0's own entering rows, at most T at a time
-    const int ro0 = min((int)T, in - s0);
-    const int p1 = min(k1, s0 + hhi);
+    // the first window: its rows outside the mesh, then its rows [k0, e0):
+    // staged in phase A and kicked at once, or loaded and kicked below
+    // sub-tile 0's own entering rows at most T at a time, sub-tile 0 kicking
+    // its own
     zero_rows(s0 - hlo, s0 + ro0 + hhi);
-    for (int a = k0; a < p1; a += (int)T) {
-        const int c = min((int)T, p1 - a);
-        load_rows(a, c, stage_a);
-        __syncthreads();
-        kick_rows(a, c);
+    // sub-tile k kicks rows [ea, eb)
+    int ea = 0, eb = 0;
+    if (early) {
+        kick_rows(k0, e0 - k0);
+    } else {
+        const int p1 = min(k1, s0 + hhi);
+        for (int a = k0; a < p1; a += (int)T) {
+            const int c = min((int)T, p1 - a);
+            load_rows(a, c, stage_a);
+            __syncthreads();
+            kick_rows(a, c);
+            __syncthreads();
+        }
+        ea = p1;
+        eb = e0;
+        if (eb > ea) {
+            load_rows(ea, eb - ea, stage_a);
+        }
         __syncthreads();
     }
-    // sub-tile k kicks rows [ea, eb): for k = 0, [p1, min(k1, s0 + ro0 + hhi))
-    int ea = p1, eb = min(k1, s0 + ro0 + hhi);
-    if (eb > ea) {
-        load_rows(ea, eb - ea, stage_a);
-    }
-    __syncthreads();
     int prev_x0 = -1, prev_rows = 0;
     for (uint32_t k = 0; k < K; k++) {
         const int x0 = s0 + (int)(k * T);
