@@ -186,6 +186,45 @@ __device__ __forceinline__ void store_row_taps(float off, uint32_t n, float* tap
     taps[4] = __int_as_float(base);
 }
 
+// The NV consecutive LDS words s[lo .. lo + NV) (lo >= 0, s 16-byte
+// aligned), handed to use(v) as v[0 .. NV), for a wavefront whose lanes all
+// have lo % 4 == sft (sft in an SGPR): 16-byte loads from s[lo - sft], and
+// use is inlined once per shift, reading the loaded registers in place.  A
+// kick's lanes start their windows 4 or 8 words apart, so NV one-word reads
+// hit a few banks many times over; the 16-byte reads of 8 neighbouring
+// lanes spread over the banks instead.  Reads up to s[lo + 4 NQ - 1], at
+// most 6 words past the window.
+template <int NV, typename F>
+__device__ __forceinline__ void lds_window16(const float* s, int32_t lo,
+                                             uint32_t sft, F&& use)
+{
+    constexpr int NQ = (NV + 6) / 4;  // sft + NV <= 4 NQ
+    const vec4f* q = reinterpret_cast<const vec4f*>(s + (lo - (int32_t)sft));
+    float w[4 * NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+        const vec4f t = q[j];
+        w[4 * j] = t.x;
+        w[4 * j + 1] = t.y;
+        w[4 * j + 2] = t.z;
+        w[4 * j + 3] = t.w;
+    }
+    switch (sft) {
+    case 0:
+        use(w);
+        break;
+    case 1:
+        use(w + 1);
+        break;
+    case 2:
+        use(w + 2);
+        break;
+    default:
+        use(w + 3);
+        break;
+    }
+}
+
 // KickMap::apply along y (src/SM/KickMap.cpp:244-267) on local rows
 // [r0, r1) of a row buffer: each thread makes four consecutive outputs of
 // a row from IT+3 consecutive inputs, taps in order (n % 4 == 0).

@@ -104,6 +104,8 @@ __host__ __device__ inline StripLds strip_lds(uint32_t n, uint32_t T, uint32_t K
     const uint32_t win_words = 2 * (nk + n) + 2 * (S + n);
     L.shared = rows_words > win_words ? rows_words : win_words;
     w += L.shared;
+    // the kick's 16-byte window reads run up to 6 words past a staged row
+    w += 8;
     L.words = w;
     return L;
 }
@@ -113,7 +115,8 @@ __host__ __device__ inline StripLds strip_lds(uint32_t n, uint32_t T, uint32_t K
 // GW consecutive outputs of a row (GW = 4 or 8; n % GW == 0) from the
 // GW + IT - 1 wake-kicked samples they read, each the same sum in the same
 // order as kick2_rows / kick_rows make it; a wavefront whose groups all
-// read inside their rows skips the per-sample checks.
+// read inside their rows skips the per-sample checks, and reads its
+// windows as 16-byte words when their starts agree modulo 4 (lds_window16).
 template <int IT, int GW>
 __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
                                            float* __restrict__ ring, uint32_t n,
@@ -149,20 +152,30 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
                                 clo + NC - 1 < in;
             float v[NV];
             float cc[NC];
-            if (__all(inside)) {
-                // every group of the wavefront lies inside its row
-#pragma unroll
-                for (int k = 0; k < NV; k++) {
-                    v[k] = s[lo + k];
-                }
+            // the wake-kicked samples of a group that lies inside its row
+            auto kick1 = [&](const float* vv) {
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
                     float acc = 0.0f;
 #pragma unroll
                     for (int j = 0; j < IT; j++) {
-                        acc = acc + v[k + j] * w1[j];
+                        acc = acc + vv[k + j] * w1[j];
                     }
                     cc[k] = acc;
+                }
+            };
+            if (__all(inside)) {
+                // every group of the wavefront lies inside its row
+                const uint32_t sft = (uint32_t)lo & 3u;
+                const uint32_t sft0 = __builtin_amdgcn_readfirstlane(sft);
+                if (__all(sft == sft0)) {
+                    lds_window16<NV>(s, lo, sft0, kick1);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NV; k++) {
+                        v[k] = s[lo + k];
+                    }
+                    kick1(v);
                 }
             } else {
 #pragma unroll
@@ -193,26 +206,34 @@ __device__ __forceinline__ void strip_kick(const float* __restrict__ stage,
             const int32_t lo = y0 + base2;
             const bool inside = lo >= 0 && lo + NC - 1 < in;
             float v[NC];
-            if (__all(inside)) {
+            auto kick2 = [&](const float* vv) {
+#pragma unroll
+                for (int e = 0; e < GW; e++) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < IT; j++) {
+                        acc = acc + vv[e + j] * w2[j];
+                    }
+                    o[e] = acc;
+                }
+            };
+            const uint32_t sft = (uint32_t)lo & 3u;
+            const uint32_t sft0 = __builtin_amdgcn_readfirstlane(sft);
+            if (__all(inside && sft == sft0)) {
+                lds_window16<NC>(s, lo, sft0, kick2);
+            } else if (__all(inside)) {
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
                     v[k] = s[lo + k];
                 }
+                kick2(v);
             } else {
 #pragma unroll
                 for (int k = 0; k < NC; k++) {
                     const uint32_t ys = (uint32_t)(lo + k);
                     v[k] = ys < n ? s[ys] : 0.0f;
                 }
-            }
-#pragma unroll
-            for (int e = 0; e < GW; e++) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int j = 0; j < IT; j++) {
-                    acc = acc + v[e + j] * w2[j];
-                }
-                o[e] = acc;
+                kick2(v);
             }
         }
         const uint32_t slot = ((uint32_t)r + hlo) & rmask;
@@ -531,14 +552,11 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
             }
         }
     };
+    // four outputs per thread: the 16-byte window reads of neighbouring
+    // lanes then fall on different banks
     auto kick_rows = [&](int a, int c) {
-        if ((n & 7u) == 0u) {
-            strip_kick<IT, 8>(stage_a, ring, n, a, c, k0, (uint32_t)hlo, rmask,
-                              tapw, tapr, wake);
-        } else {
-            strip_kick<IT, 4>(stage_a, ring, n, a, c, k0, (uint32_t)hlo, rmask,
-                              tapw, tapr, wake);
-        }
+        strip_kick<IT, 4>(stage_a, ring, n, a, c, k0, (uint32_t)hlo, rmask,
+                          tapw, tapr, wake);
     };
     // ring rows of [lo, hi) outside the mesh read as zeros
     auto zero_rows = [&](int lo, int hi) {
