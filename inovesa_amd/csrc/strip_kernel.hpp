@@ -4,7 +4,8 @@
 // instance, in K sub-tiles of T rows, instead of one tile of T rows:
 //   A  once per strip: the previous x projection, the per-column drift,
 //      Fokker-Planck and Simpson tables, the RF taps of every row the strip
-//      kicks;
+//      kicks and (when they fit the staging rows) the rows the first drift
+//      window kicks, all loads issued before any store;
 //   B  once per strip: the direct wake offsets of the kicked rows and the
 //      CSR dot products of the output rows, STRIP_DOT_ROWS rows per
 //      wavefront sharing each load of the profile (each row's sum keeps the
