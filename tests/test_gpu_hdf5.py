@@ -161,6 +161,32 @@ def test_renormalize_batch_equals_single(V, strip):
             assert np.array_equal(x, y), k
 
 
+def test_renormalize_strip_with_staged_first_window(V):
+    """Strips of 8 tiles of 4 rows at 512^2: the rows of each strip's first
+    drift window are staged before the renormalisation factor is known
+    (their halo fits the two staging tiles) and scaled in place afterwards;
+    every step still equals the single step's.  2000 steps per synchrotron
+    period, as in the 512^2 benchmark workload, keep the drift's halo within
+    a tile."""
+    from inovesa_amd.workloads import impedances, initial_mesh, kick_drift_config
+    desc, d = kick_drift_config(n=512, steps_per_ts=2000)
+    zw, zc = impedances(d)
+    init = initial_mesh(512, seed=9)
+    desc = dict(desc, renormalize=2)
+    inits = [init, init * np.float32(0.8)]
+    b = V.Batch([desc, desc], inits, [zw, zw], [zc, zc])
+    b.retile_size(512, 4, 512, strip=8)
+    cfg = b.config(0)
+    assert cfg["strip"] == 8 and cfg["tile_rows"] == 4
+    assert cfg["stride"] - cfg["tile_rows"] <= cfg["tile_rows"]  # halo <= T
+    b.run(9)
+    for k, x0 in enumerate(inits):
+        s = V.Step(desc, x0, zw, zc)
+        s.run(9)
+        for x, y in zip(b.mesh(k), s.mesh()):
+            assert np.array_equal(x, y), k
+
+
 def test_renormalize_needs_direct_fused_step(V):
     desc, d, zw, zc, init = _setup(n=128)
     with pytest.raises(V.VfpsInvalid):
