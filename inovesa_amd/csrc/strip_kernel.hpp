@@ -100,9 +100,11 @@ __host__ __device__ inline StripLds strip_lds(uint32_t n, uint32_t T, uint32_t K
     L.stage_a = L.ring + strip_ring_rows(T, halo) * n4;
     L.stage_b = L.stage_a + T * n4;
     const uint32_t rows_words = L.stage_b + T * n4 - L.ring;
+    // (each window is followed by STRIP_DOT_ROWS doubles that the dot
+    // products' unused row slots may read)
     L.gwin = w;
-    L.hwin = L.gwin + 2 * (nk + n);
-    const uint32_t win_words = 2 * (nk + n) + 2 * (S + n);
+    L.hwin = L.gwin + 2 * (nk + n + STRIP_DOT_ROWS);
+    const uint32_t win_words = 2 * (nk + n + STRIP_DOT_ROWS) + 2 * (S + n + STRIP_DOT_ROWS);
     L.shared = rows_words > win_words ? rows_words : win_words;
     w += L.shared;
     // the kick's 16-byte window reads run up to 6 words past a staged row
@@ -324,7 +326,7 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
     const uint32_t etot4 = (uint32_t)(e0 - k0) * n4;
     const bool early = e0 - k0 <= 2 * (int)T &&
                        etot4 <= (uint32_t)STRIP_EARLY4 * bs &&
-                       L.hwin + 2u * (S + n) <= L.stage_a;
+                       L.hwin + 2u * (S + n + STRIP_DOT_ROWS) <= L.stage_a;
     vec4f er[STRIP_EARLY4];
     if (early) {
         const gfloat4* src4 = (const gfloat4*)(in_mesh + (size_t)k0 * n);
@@ -472,12 +474,21 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
             for (int i = 0; i < STRIP_DOT_ROWS; i++) {
                 acc[i] = 0.0;
             }
+            // every row slot reads its window value (past a job's last row
+            // they read words the windows' padding keeps inside LDS, and
+            // their sums are dropped), so that the loads of one x' go out
+            // together instead of one branch and one wait per row
             for (uint32_t x = lane; x < n; x += 64) {
                 const double p = (double)pprev[x];
                 const double* t = tbl - x;
+                double tv[STRIP_DOT_ROWS];
 #pragma unroll
                 for (int i = 0; i < STRIP_DOT_ROWS; i++) {
-                    if (i < rows) acc[i] += t[i] * p;
+                    tv[i] = t[i];
+                }
+#pragma unroll
+                for (int i = 0; i < STRIP_DOT_ROWS; i++) {
+                    acc[i] += tv[i] * p;
                 }
             }
             double* dst = isw ? wsum + (r_first - k0) : csum + (r_first - s0);
