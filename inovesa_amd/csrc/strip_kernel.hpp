@@ -646,14 +646,15 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
             const bool win_ok = ((dcons >> c) & 1u) && r0 >= wlo &&
                                 r0 + rows_out + IT - 2 < whi;
             auto window = [&]() {
+                // every slot is read, so that the loads go out together
+                // (one past the window reads some ring row, in bounds, and
+                // is not used)
                 float win[STRIP_MAX_T + IT - 1];
 #pragma unroll
                 for (int t = 0; t < (int)STRIP_MAX_T + IT - 1; t++) {
-                    if (t < rows_out + IT - 1) {
-                        const uint32_t slot =
-                            ((uint32_t)(r0 + t) + (uint32_t)hlo) & rmask;
-                        win[t] = ring[__umul24(slot, n) + y];
-                    }
+                    const uint32_t slot =
+                        ((uint32_t)(r0 + t) + (uint32_t)hlo) & rmask;
+                    win[t] = ring[__umul24(slot, n) + y];
                 }
 #pragma unroll
                 for (int o = 0; o < (int)STRIP_MAX_T; o++) {
