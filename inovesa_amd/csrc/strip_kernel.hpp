@@ -588,6 +588,28 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
             }
             const bool fc = (fcons >> c) & 1u;
             const uint32_t f0 = col_f0[c];
+            if (IPFP > 0 && rows_out == 4 && __all(fc)) {
+                // a full tile of four rows: every stencil value loads
+                // before the sums (the same sums in the same order)
+                float sv[4][IPFP > 0 ? IPFP : 1];
+#pragma unroll
+                for (int o = 0; o < 4; o++) {
+#pragma unroll
+                    for (int kk = 0; kk < IPFP; kk++) {
+                        sv[o][kk] = stage_b[(size_t)o * n + f0 + (uint32_t)kk];
+                    }
+                }
+#pragma unroll
+                for (int o = 0; o < 4; o++) {
+                    float v = 0.0f;
+#pragma unroll
+                    for (int kk = 0; kk < IPFP; kk++) {
+                        v = v + sv[o][kk] * col_fw[c][kk];
+                    }
+                    out[(size_t)(x0 + o) * n + y] = v;
+                }
+                continue;
+            }
             if (IPFP == 0 || __all(fc)) {
                 // every column of the wavefront has a consecutive stencil
                 for (int o = 0; o < rows_out; o++) {
