@@ -362,6 +362,22 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
     uint32_t col_f0[MAXC];
     float col_fw[MAXC][IPFP > 0 ? IPFP : 1];
     uint32_t dcons = 0, fcons = 0;  // bit c: column c's taps are consecutive
+    // their loads (and the first RF offset per thread) go out before any
+    // is used; a column past the mesh reads column 0, unused
+    float col_doff[MAXC];
+    uint32_t col_fi[MAXC][IPFP > 0 ? IPFP : 1];
+#pragma unroll
+    for (int c = 0; c < MAXC; c++) {
+        const uint32_t y = tid + (uint32_t)c * bs;
+        const uint32_t yy = (c < ncols && y < n) ? y : 0u;
+        col_doff[c] = drift_off[yy];
+#pragma unroll
+        for (int k = 0; k < IPFP; k++) {
+            col_fi[c][k] = fp_i[(size_t)k * n + yy];
+            col_fw[c][k] = fp_w[(size_t)k * n + yy];
+        }
+    }
+    const float rf0 = rf_off[k0 + min((int)tid, nk - 1)];
 #pragma unroll
     for (int c = 0; c < MAXC; c++) {
         const uint32_t y = tid + (uint32_t)c * bs;
@@ -369,7 +385,7 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
         col_f0[c] = 0;
         if (c < ncols && y < n) {
             uint32_t idx[IT];
-            kick_taps<IT>(drift_off[y], n, idx, col_dw[c]);
+            kick_taps<IT>(col_doff[c], n, idx, col_dw[c]);
             bool cons = true;
 #pragma unroll
             for (int j = 1; j < IT; j++) {
@@ -380,15 +396,17 @@ __device__ __forceinline__ void strip_body(const InstDesc& d, uint32_t strip,
             bool fc = true;
 #pragma unroll
             for (int k = 0; k < IPFP; k++) {
-                const uint32_t fi = fp_i[(size_t)k * n + y];
-                col_fw[c][k] = fp_w[(size_t)k * n + y];
+                const uint32_t fi = col_fi[c][k];
                 if (k == 0) col_f0[c] = fi;
                 fc = fc && fi == col_f0[c] + (uint32_t)k;
             }
             fcons |= fc ? (1u << c) : 0u;
         }
     }
-    for (int i = (int)tid; i < nk; i += (int)bs) {
+    if ((int)tid < nk) {
+        store_row_taps<IT>(rf0, n, tapr + tid * STEP_TAP_WORDS);
+    }
+    for (int i = (int)(tid + bs); i < nk; i += (int)bs) {
         store_row_taps<IT>(rf_off[k0 + i], n, tapr + (uint32_t)i * STEP_TAP_WORDS);
     }
 #pragma unroll
