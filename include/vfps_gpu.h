@@ -570,7 +570,7 @@ VFPS_API int vfps_batch_timing(vfps_batch* b, uint32_t nsteps, float* ms_step);
 /* Autotuner: per mesh size, time every tile height T in {1,2,4,8,16},
  * workgroup size in {128,256,512} and launch form (one workgroup per tile;
  * persistent: as many workgroups as stay resident, each walking several
- * tiles; for the kick-drift maps on meshes of 512 rows or more also
+ * tiles; for the kick-drift maps on meshes of 1024 rows or more also
  * strips of 16, 32 or 64 rows in tiles of 2, 4 or 8) that fits (steps steps each, on that size's
  * instances alone), time the three fastest again over 4 x steps and keep
  * the fastest of those.  The instances' state is restored afterwards;

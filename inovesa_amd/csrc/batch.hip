@@ -798,12 +798,14 @@ int vfps_batch_autotune(vfps_batch* b, uint32_t steps, float* table,
             orig.push_back({m->T, m->threads, m->strip, m->persist});
         }
         // strips exist for the kick-drift maps with the direct wake; below
-        // 512 rows a size class has too few strips to fill the GPU, so the
-        // tuner does not spend time on them there
+        // 1024 rows a size class has too few strips to fill the GPU beside
+        // the other classes of a batch (at 512 rows strips timed alone can
+        // beat tiles, yet the whole ragged batch then runs slower), so the
+        // tuner does not try them there
         bool direct = true;  // the resident-grid form needs every member
         for (uint32_t i : cl.second) direct = direct && b->members[i]->direct;
         const bool strips_ok =
-            n >= 512 &&
+            n >= 1024 &&
             strip_kernel_for(b->members[cl.second[0]]->variant, 1) != nullptr &&
             !b->members[cl.second[0]]->desc.fast_projection;
         std::vector<Cand> cands;
